@@ -1,0 +1,645 @@
+// kg_api.hip -- the C ABI of include/kobayashi_gpu.h over the gfx950 kernels.
+//
+// A handle owns: device fields phi[2], T[2] (ping-pong), theta (in place),
+// pass-1 temporaries for the two-pass kernel (allocated on first use), FP64
+// reduction scratch, one non-blocking HIP stream at default priority and two
+// timing events.  Host code here validates every argument before any launch,
+// so a kernel never sees a size or pointer it was not written for.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kobayashi_gpu.h"
+#include "kg_internal.h"
+#include "kg_philox.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(KG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define KG_HIP(call)                               \
+    do {                                           \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess)                      \
+            return hip_fail(e_, #call);            \
+    } while (0)
+
+constexpr long long kMaxCells = 1ll << 32;  // 4 Gi cells (64 GiB per FP32 field pair)
+
+}  // namespace
+
+struct kg_solver_s {
+    int nx = 0, ny = 0, precision = KG_F32;
+    kg_params params{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *phi[2] = {nullptr, nullptr};
+    void *T[2] = {nullptr, nullptr};
+    void *theta = nullptr;
+    void *temps = nullptr;  // 6 fields, two-pass kernel only
+    double *partials = nullptr;
+    kg::ReduceOut *red = nullptr;
+    int *disks = nullptr;
+    int disks_cap = 0;
+    int cur = 0;
+    uint64_t steps = 0;
+    int kernel = KG_KERNEL_FUSED;
+
+    size_t cells() const { return (size_t)nx * (size_t)ny; }
+    size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
+    size_t field_bytes() const { return cells() * esize(); }
+};
+
+namespace {
+
+bool finite_pos(double v) { return std::isfinite(v) && v > 0.0; }
+
+int check_handle(kg_handle h)
+{
+    if (!h)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null handle");
+    return KG_OK;
+}
+
+template <typename R>
+kg::StepArgs<R> make_args(const kg_solver_s &h)
+{
+    const kg_params &p = h.params;
+    kg::StepArgs<R> a{};
+    a.phi_in = static_cast<const R *>(h.phi[h.cur]);
+    a.T_in = static_cast<const R *>(h.T[h.cur]);
+    a.phi_out = static_cast<R *>(h.phi[h.cur ^ 1]);
+    a.T_out = static_cast<R *>(h.T[h.cur ^ 1]);
+    a.theta = static_cast<R *>(h.theta);
+    a.nx = h.nx;
+    a.ny = h.ny;
+    a.boundary = p.boundary;
+    // scalar combinations in R, in the reference's association order
+    const R dx = (R)p.dx, dy = (R)p.dy, dt = (R)p.dt, tau = (R)p.tau;
+    a.inv_dx = (R)1 / dx;
+    a.inv_dy = (R)1 / dy;
+    a.inv_3dx2 = (R)1 / ((R)3 * dx * dx);
+    a.dt = dt;
+    a.dt_tau = dt / tau;
+    a.eps_bar = (R)p.eps_bar;
+    a.delta = (R)p.delta;
+    a.aniso = (R)p.aniso;
+    a.theta0 = (R)p.theta0;
+    a.neg_ebar_j_delta = -(R)p.eps_bar * (R)p.aniso * (R)p.delta;
+    a.alpha_pi = (R)p.alpha / (sizeof(R) == 4 ? (R)3.14159274f : (R)3.14159265358979323846);
+    a.gamma = (R)p.gamma;
+    a.teq = (R)p.teq;
+    a.K = (R)p.K;
+    a.noise_a = (R)p.noise_a;
+    a.instance = p.instance;
+    a.key0 = (uint32_t)p.seed;
+    a.key1 = (uint32_t)(p.seed >> 32);
+    return a;
+}
+
+int ensure_temps(kg_solver_s &h)
+{
+    if (h.temps)
+        return KG_OK;
+    hipError_t e = hipMalloc(&h.temps, 6 * h.field_bytes());
+    if (e != hipSuccess) {
+        h.temps = nullptr;
+        return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of two-pass temporaries failed");
+    }
+    return KG_OK;
+}
+
+template <typename R>
+int step_impl(kg_solver_s &h, int nsteps)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        kg::StepArgs<R> a = make_args<R>(h);
+        a.step = (uint32_t)h.steps;
+        hipError_t e;
+        if (h.kernel == KG_KERNEL_TWO_PASS) {
+            R *base = static_cast<R *>(h.temps);
+            const size_t n = h.cells();
+            kg::PassTemps<R> t{base, base + n, base + 2 * n, base + 3 * n, base + 4 * n,
+                               base + 5 * n};
+            e = kg::launch_step_two_pass<R>(a, t, h.stream);
+        } else {
+            e = kg::launch_step_fused<R>(a, h.stream);
+        }
+        if (e != hipSuccess)
+            return hip_fail(e, "step launch");
+        h.cur ^= 1;
+        h.steps += 1;
+    }
+    return KG_OK;
+}
+
+int init_fields(kg_solver_s &h, int kind, int x, int y, const std::vector<int> &disks, double r2)
+{
+    KG_HIP(hipSetDevice(h.device));
+    const int nd = (int)disks.size() / 2;
+    if (nd > h.disks_cap) {
+        if (h.disks)
+            KG_HIP(hipFree(h.disks));
+        h.disks = nullptr;
+        KG_HIP(hipMalloc(&h.disks, disks.size() * sizeof(int)));
+        h.disks_cap = nd;
+    }
+    if (nd > 0)
+        KG_HIP(hipMemcpyAsync(h.disks, disks.data(), disks.size() * sizeof(int),
+                              hipMemcpyHostToDevice, h.stream));
+    hipError_t e;
+    h.cur = 0;
+    if (h.precision == KG_F64)
+        e = kg::launch_init<double>(static_cast<double *>(h.phi[0]), static_cast<double *>(h.T[0]),
+                                    static_cast<double *>(h.theta), h.nx, h.ny, kind, x, y, h.disks,
+                                    nd, r2, h.stream);
+    else
+        e = kg::launch_init<float>(static_cast<float *>(h.phi[0]), static_cast<float *>(h.T[0]),
+                                   static_cast<float *>(h.theta), h.nx, h.ny, kind, x, y, h.disks,
+                                   nd, r2, h.stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "init launch");
+    KG_HIP(hipStreamSynchronize(h.stream));
+    h.steps = 0;
+    return KG_OK;
+}
+
+void free_all(kg_solver_s *h)
+{
+    if (!h)
+        return;
+    (void)hipSetDevice(h->device);
+    if (h->stream)
+        (void)hipStreamSynchronize(h->stream);
+    void *bufs[] = {h->phi[0], h->phi[1], h->T[0], h->T[1], h->theta, h->temps, h->partials,
+                    h->red, h->disks};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+    if (h->ev0)
+        (void)hipEventDestroy(h->ev0);
+    if (h->ev1)
+        (void)hipEventDestroy(h->ev1);
+    if (h->stream)
+        (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int copy_field_out(kg_solver_s *h, const void *src, void *dst, size_t bytes)
+{
+    if (!dst)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null destination");
+    if (bytes != h->field_bytes())
+        return fail(KG_ERR_SIZE, "buffer size " + std::to_string(bytes) + " != field size " +
+                                     std::to_string(h->field_bytes()));
+    KG_HIP(hipSetDevice(h->device));
+    KG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+    KG_HIP(hipStreamSynchronize(h->stream));
+    return KG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kg_abi_version(void) { return KG_ABI_VERSION; }
+
+const char *kg_last_error(void) { return g_last_error.c_str(); }
+
+int kg_device_count(int *count)
+{
+    if (!count)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        n = 0;
+    *count = n;
+    return KG_OK;
+}
+
+int kg_default_params(kg_params *out)
+{
+    if (!out)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null params");
+    std::memset(out, 0, sizeof(*out));
+    out->dx = (double)0.03f;
+    out->dy = (double)0.03f;
+    out->dt = (double)0.0001f;
+    out->tau = (double)0.0003f;
+    out->eps_bar = (double)0.010f;
+    out->mu = (double)1.0f;
+    out->K = (double)1.6f;
+    out->delta = (double)0.05f;
+    out->aniso = (double)6.0f;
+    out->alpha = (double)0.9f;
+    out->gamma = (double)10.0f;
+    out->teq = (double)1.0f;
+    out->theta0 = 0.0;
+    out->noise_a = 0.0;
+    out->boundary = KG_PERIODIC;
+    out->instance = 0;
+    out->seed = 0;
+    return KG_OK;
+}
+
+int kg_validate(int nx, int ny, const kg_params *p)
+{
+    if (!p)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null params");
+    if (nx < 3 || ny < 3)
+        return fail(KG_ERR_INVALID_ARGUMENT, "grid must be at least 3x3");
+    if ((long long)nx * (long long)ny > kMaxCells)
+        return fail(KG_ERR_INVALID_ARGUMENT, "grid larger than 2^32 cells");
+    if (!finite_pos(p->dx) || !finite_pos(p->dy) || !finite_pos(p->dt) || !finite_pos(p->tau) ||
+        !finite_pos(p->eps_bar))
+        return fail(KG_ERR_INVALID_ARGUMENT, "dx, dy, dt, tau and eps_bar must be finite and > 0");
+    const double vals[] = {p->mu, p->K, p->delta, p->aniso, p->alpha, p->gamma,
+                           p->teq, p->theta0, p->noise_a};
+    for (double v : vals)
+        if (!std::isfinite(v))
+            return fail(KG_ERR_INVALID_ARGUMENT, "non-finite parameter");
+    if (p->boundary != KG_PERIODIC && p->boundary != KG_ZERO_FLUX)
+        return fail(KG_ERR_INVALID_ARGUMENT, "boundary must be KG_PERIODIC or KG_ZERO_FLUX");
+    // forward Euler with the 9-point stencil: the most negative symbol is
+    // -16/(3 dx^2) (checkerboard), stable while dt * 16/(3 dx^2) <= 2
+    const double h2 = std::fmin(p->dx, p->dy) * std::fmin(p->dx, p->dy);
+    if (p->dt / h2 > 0.375)
+        return fail(KG_ERR_UNSTABLE, "dt/dx^2 exceeds 3/8 (explicit-Euler bound of the heat equation)");
+    const double d = 1.0 + std::fabs(p->delta);
+    if (p->dt * p->eps_bar * p->eps_bar * d * d / (p->tau * h2) > 0.375)
+        return fail(KG_ERR_UNSTABLE, "dt eps_bar^2 (1+|delta|)^2 / (tau dx^2) exceeds 3/8");
+    return KG_OK;
+}
+
+int kg_reference_range_mask(const kg_params *p, uint32_t *mask)
+{
+    if (!p || !mask)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null argument");
+    // slider {min, max} of src/Kobayashi.cpp:17-58
+    const double v[9] = {p->tau, p->eps_bar, p->mu, p->K, p->delta,
+                         p->aniso, p->alpha, p->gamma, p->teq};
+    const double lo[9] = {0.0001f, 0.006f, 0.5f, 1.0f, 0.01f, 2.0f, 0.7f, 10.0f, 0.5f};
+    const double hi[9] = {0.0009f, 0.015f, 1.4f, 1.9f, 0.09f, 8.0f, 1.2f, 20.0f, 1.5f};
+    uint32_t m = 0;
+    for (int k = 0; k < 9; ++k)
+        if (!(v[k] >= lo[k] - 1e-9 && v[k] <= hi[k] + 1e-9))
+            m |= 1u << k;
+    *mask = m;
+    return KG_OK;
+}
+
+int kg_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    if (!ctr || !key || !out)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null argument");
+    const kg::u32x4 r = kg::philox4x32_10(kg::u32x4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+    return KG_OK;
+}
+
+int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out)
+{
+    if (!out)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (precision != KG_F32 && precision != KG_F64)
+        return fail(KG_ERR_INVALID_ARGUMENT, "precision must be KG_F32 or KG_F64");
+    int rc = kg_validate(nx, ny, p);
+    if (rc != KG_OK)
+        return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(KG_ERR_DEVICE, "no GPU visible to the HIP runtime");
+    kg_solver_s *h = new kg_solver_s();
+    h->nx = nx;
+    h->ny = ny;
+    h->precision = precision;
+    h->params = *p;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&h->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&h->ev1);
+    if (e != hipSuccess) {
+        free_all(h);
+        return hip_fail(e, "stream/event creation");
+    }
+    void **bufs[] = {&h->phi[0], &h->phi[1], &h->T[0], &h->T[1], &h->theta};
+    for (void **b : bufs) {
+        if (hipMalloc(b, h->field_bytes()) != hipSuccess) {
+            *b = nullptr;
+            free_all(h);
+            return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the fields failed");
+        }
+    }
+    if (hipMalloc(&h->partials, kg::reduce_partials_size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h->red, sizeof(kg::ReduceOut)) != hipSuccess) {
+        free_all(h);
+        return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of reduction scratch failed");
+    }
+    rc = kg_reset(h);
+    if (rc != KG_OK) {
+        free_all(h);
+        return rc;
+    }
+    *out = h;
+    return KG_OK;
+}
+
+int kg_destroy(kg_handle h)
+{
+    if (!h)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null handle");
+    free_all(h);
+    return KG_OK;
+}
+
+int kg_reset(kg_handle h)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    return init_fields(*h, 0, h->nx / 2, h->ny / 2, {}, 0.0);
+}
+
+int kg_reset_disk(kg_handle h, int cx, int cy, double r)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!(r >= 0.0) || !std::isfinite(r))
+        return fail(KG_ERR_INVALID_ARGUMENT, "radius must be finite and >= 0");
+    return init_fields(*h, 1, 0, 0, {cx, cy}, r * r);
+}
+
+int kg_reset_seeds(kg_handle h, int count, double r, int margin, uint64_t seed, int *cxy)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (count < 0 || count > 1 << 16)
+        return fail(KG_ERR_INVALID_ARGUMENT, "count must be in [0, 65536]");
+    if (!(r >= 0.0) || !std::isfinite(r))
+        return fail(KG_ERR_INVALID_ARGUMENT, "radius must be finite and >= 0");
+    if (margin < 0 || 2 * margin >= h->nx || 2 * margin >= h->ny)
+        return fail(KG_ERR_INVALID_ARGUMENT, "margin leaves no room for centres");
+    const uint32_t spanx = (uint32_t)(h->nx - 2 * margin), spany = (uint32_t)(h->ny - 2 * margin);
+    std::vector<int> d(2 * (size_t)count);
+    for (int k = 0; k < count; ++k) {
+        // counter word 3 = 0xFFFFFFFF keeps these draws apart from the noise
+        // counters (word 3 = ensemble member id)
+        const kg::u32x4 v = kg::philox4x32_10(kg::u32x4{(uint32_t)k, 0u, 0u, 0xFFFFFFFFu},
+                                              (uint32_t)seed, (uint32_t)(seed >> 32));
+        d[2 * k] = margin + (int)(v.x % spanx);
+        d[2 * k + 1] = margin + (int)(v.y % spany);
+    }
+    if (cxy)
+        std::memcpy(cxy, d.data(), d.size() * sizeof(int));
+    return init_fields(*h, 1, 0, 0, d, r * r);
+}
+
+int kg_set_params(kg_handle h, const kg_params *p)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (int rc = kg_validate(h->nx, h->ny, p))
+        return rc;
+    h->params = *p;
+    return KG_OK;
+}
+
+int kg_get_params(kg_handle h, kg_params *p)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!p)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null params");
+    *p = h->params;
+    return KG_OK;
+}
+
+int kg_get_size(kg_handle h, int *nx, int *ny, int *precision)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (nx)
+        *nx = h->nx;
+    if (ny)
+        *ny = h->ny;
+    if (precision)
+        *precision = h->precision;
+    return KG_OK;
+}
+
+int kg_step(kg_handle h, int nsteps)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (nsteps < 0)
+        return fail(KG_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+    KG_HIP(hipSetDevice(h->device));
+    if (h->kernel == KG_KERNEL_TWO_PASS)
+        if (int rc = ensure_temps(*h))
+            return rc;
+    return h->precision == KG_F64 ? step_impl<double>(*h, nsteps) : step_impl<float>(*h, nsteps);
+}
+
+int kg_step_timed(kg_handle h, int nsteps, double *ms)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!ms)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null ms");
+    KG_HIP(hipSetDevice(h->device));
+    KG_HIP(hipEventRecord(h->ev0, h->stream));
+    if (int rc = kg_step(h, nsteps))
+        return rc;
+    KG_HIP(hipEventRecord(h->ev1, h->stream));
+    KG_HIP(hipEventSynchronize(h->ev1));
+    float t = 0.0f;
+    KG_HIP(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    *ms = (double)t;
+    return KG_OK;
+}
+
+int kg_synchronize(kg_handle h)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    KG_HIP(hipSetDevice(h->device));
+    KG_HIP(hipStreamSynchronize(h->stream));
+    return KG_OK;
+}
+
+int kg_get_step_count(kg_handle h, uint64_t *steps)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!steps)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null steps");
+    *steps = h->steps;
+    return KG_OK;
+}
+
+int kg_set_step_count(kg_handle h, uint64_t steps)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    h->steps = steps;
+    return KG_OK;
+}
+
+int kg_set_kernel(kg_handle h, int kernel)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (kernel != KG_KERNEL_FUSED && kernel != KG_KERNEL_TWO_PASS)
+        return fail(KG_ERR_INVALID_ARGUMENT, "unknown kernel");
+    h->kernel = kernel;
+    return KG_OK;
+}
+
+int kg_get_phase(kg_handle h, void *dst, size_t bytes)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    return copy_field_out(h, h->phi[h->cur], dst, bytes);
+}
+
+int kg_get_temperature(kg_handle h, void *dst, size_t bytes)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    return copy_field_out(h, h->T[h->cur], dst, bytes);
+}
+
+int kg_get_angle(kg_handle h, void *dst, size_t bytes)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    return copy_field_out(h, h->theta, dst, bytes);
+}
+
+int kg_set_fields(kg_handle h, const void *phi, const void *T, const void *theta, size_t bytes)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!phi || !T)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null phi or T");
+    if (bytes != h->field_bytes())
+        return fail(KG_ERR_SIZE, "buffer size does not match the field size");
+    KG_HIP(hipSetDevice(h->device));
+    KG_HIP(hipMemcpyAsync(h->phi[h->cur], phi, bytes, hipMemcpyHostToDevice, h->stream));
+    KG_HIP(hipMemcpyAsync(h->T[h->cur], T, bytes, hipMemcpyHostToDevice, h->stream));
+    if (theta)
+        KG_HIP(hipMemcpyAsync(h->theta, theta, bytes, hipMemcpyHostToDevice, h->stream));
+    else
+        KG_HIP(hipMemsetAsync(h->theta, 0, bytes, h->stream));
+    KG_HIP(hipStreamSynchronize(h->stream));
+    return KG_OK;
+}
+
+int kg_device_fields(kg_handle h, void **phi, void **T, void **theta)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (phi)
+        *phi = h->phi[h->cur];
+    if (T)
+        *T = h->T[h->cur];
+    if (theta)
+        *theta = h->theta;
+    return KG_OK;
+}
+
+int kg_get_stream(kg_handle h, void **stream)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!stream)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null stream");
+    *stream = (void *)h->stream;
+    return KG_OK;
+}
+
+int kg_reduce(kg_handle h, kg_stats *out)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!out)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null stats");
+    KG_HIP(hipSetDevice(h->device));
+    hipError_t e;
+    if (h->precision == KG_F64)
+        e = kg::launch_reduce<double>(static_cast<const double *>(h->phi[h->cur]),
+                                      static_cast<const double *>(h->T[h->cur]), h->cells(),
+                                      h->params.K, h->partials, h->red, h->stream);
+    else
+        e = kg::launch_reduce<float>(static_cast<const float *>(h->phi[h->cur]),
+                                     static_cast<const float *>(h->T[h->cur]), h->cells(),
+                                     (double)(float)h->params.K, h->partials, h->red, h->stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "reduce launch");
+    kg::ReduceOut r{};
+    KG_HIP(hipMemcpyAsync(&r, h->red, sizeof(r), hipMemcpyDeviceToHost, h->stream));
+    KG_HIP(hipStreamSynchronize(h->stream));
+    out->sum_phi = r.sum_phi;
+    out->sum_T = r.sum_T;
+    out->sum_T_minus_K_phi = r.sum_T_minus_K_phi;
+    out->min_phi = r.min_phi;
+    out->max_phi = r.max_phi;
+    out->solid_fraction = r.above_half / (double)h->cells();
+    out->nonfinite = (int64_t)r.nonfinite;
+    out->cells = (int64_t)h->cells();
+    return KG_OK;
+}
+
+int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!rgba)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null rgba");
+    if (order != 0 && order != 1)
+        return fail(KG_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
+    if (order == 1 && h->nx != h->ny)
+        return fail(KG_ERR_INVALID_ARGUMENT, "the reference's buffer order needs a square grid");
+    if (bytes != h->cells() * 4 * sizeof(float))
+        return fail(KG_ERR_SIZE, "rgba buffer must hold 4 floats per cell");
+    KG_HIP(hipSetDevice(h->device));
+    float *d = nullptr;
+    KG_HIP(hipMalloc(&d, bytes));
+    hipError_t e = h->precision == KG_F64
+                       ? kg::launch_colormap<double>(static_cast<const double *>(h->phi[h->cur]),
+                                                     h->nx, h->ny, order, d, h->stream)
+                       : kg::launch_colormap<float>(static_cast<const float *>(h->phi[h->cur]),
+                                                    h->nx, h->ny, order, d, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rgba, d, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return hip_fail(e, "colormap");
+    return KG_OK;
+}
+
+}  // extern "C"

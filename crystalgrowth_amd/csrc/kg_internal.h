@@ -1,0 +1,73 @@
+// kg_internal.h -- arguments of the step kernels and the host-side launchers
+// shared by kg_step.hip (device code) and kg_api.hip (the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace kg {
+
+// Everything one explicit-Euler step needs, with the reference's scalar
+// parameters pre-combined in the field's precision R the way the reference
+// combines them (src/Kobayashi.cpp:139-215):
+//   gx = (phiE - phiW) / dx            -> (phiE - phiW) * inv_dx
+//   lap = (...) / (3 dx dx)            -> (...) * inv_3dx2
+//   eps' = ((-eps_bar) j) delta sin()  -> neg_ebar_j_delta * sin()
+//   m = alpha / pi * atan(...)         -> alpha_pi * atan(...)
+//   phi' = phi + (...) dt / tau        -> phi + (...) * dt_tau
+template <typename R>
+struct StepArgs {
+    const R *phi_in;
+    const R *T_in;
+    R *phi_out;
+    R *T_out;
+    R *theta;        // persistent angle, updated in place where a branch fires
+    int nx, ny;
+    int boundary;    // 0 periodic, 1 zero-flux
+    int ntiles;      // tiles of the grid
+    int per_xcd;     // tiles per XCD slot of the block-id remap
+    R inv_dx, inv_dy, inv_3dx2, dt, dt_tau;
+    R eps_bar, delta, aniso, theta0, neg_ebar_j_delta;
+    R alpha_pi, gamma, teq, K, noise_a;
+    uint32_t step, instance, key0, key1;
+};
+
+// Two-pass variant: pass-1 temporaries in global memory, as the reference's
+// member arrays (src/Kobayashi.h:109-114).
+template <typename R>
+struct PassTemps {
+    R *gx, *gy, *lap, *lapT, *eps, *epsd;
+};
+
+struct ReduceOut {
+    double sum_phi, sum_T, sum_T_minus_K_phi, min_phi, max_phi, above_half;
+    double nonfinite;
+};
+
+template <typename R>
+hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s);
+template <typename R>
+hipError_t launch_step_two_pass(const StepArgs<R> &a, const PassTemps<R> &t, hipStream_t s);
+
+// Initial state: all fields zero, then phi = 1 on the reference's plus at
+// (x, y) (kind 0) or on `ndisks` disks (kind 1; disks holds cx, cy pairs in
+// device memory, r2 the squared radius).
+template <typename R>
+hipError_t launch_init(R *phi, R *T, R *theta, int nx, int ny, int kind, int x, int y,
+                       const int *disks, int ndisks, double r2, hipStream_t s);
+
+// Deterministic FP64 reduction; `partials` holds reduce_partials_size()
+// doubles, `out` one ReduceOut (both device memory).
+size_t reduce_partials_size();
+template <typename R>
+hipError_t launch_reduce(const R *phi, const R *T, size_t n, double K, double *partials,
+                         ReduceOut *out, hipStream_t s);
+
+template <typename R>
+hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba, hipStream_t s);
+
+// Tiles of the fused kernel and its launch geometry.
+void fused_geometry(int nx, int ny, int *ntiles, int *per_xcd, int *nblocks);
+
+}  // namespace kg

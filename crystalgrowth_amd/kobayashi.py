@@ -1,0 +1,214 @@
+"""Python host API over the C ABI.
+
+`Solver` is the general handle (any size, FP32/FP64, periodic or zero-flux,
+noise, seeded initial states).  `Kobayashi` keeps the reference class's
+simulation surface (src/Kobayashi.h:35-59): `Kobayashi(x, y, timeStep)`,
+`iUpdate()` = 10 steps (src/Kobayashi.cpp:227-239), `iResetSimulationState()`,
+`iGetObjectCount()`, `iIsUpdated()`, `iUpdateConstantBuffer()` colours
+(:309-345), plus the phase/temperature/angle accessors the headless use needs.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import KG_F32, KG_F64, KG_KERNEL_FUSED, KG_KERNEL_TWO_PASS, Params, Stats, check, lib
+
+_PREC = {"f32": KG_F32, "float32": KG_F32, "f64": KG_F64, "float64": KG_F64}
+_KERNELS = {"fused": KG_KERNEL_FUSED, "two_pass": KG_KERNEL_TWO_PASS}
+
+
+def default_params(**overrides) -> Params:
+    """Reference defaults (src/Kobayashi.cpp:61-63, :76-84, dt of src/main.cpp:16)."""
+    p = Params()
+    check(lib().kg_default_params(ctypes.byref(p)))
+    return p.replace(**overrides) if overrides else p
+
+
+def validate(nx: int, ny: int, params: Params) -> None:
+    check(lib().kg_validate(nx, ny, ctypes.byref(params)))
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    check(lib().kg_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def philox4x32_10(ctr, key) -> np.ndarray:
+    c = (ctypes.c_uint32 * 4)(*[int(x) for x in ctr])
+    k = (ctypes.c_uint32 * 2)(*[int(x) for x in key])
+    o = (ctypes.c_uint32 * 4)()
+    check(lib().kg_philox4x32_10(c, k, o))
+    return np.array(list(o), dtype=np.uint32)
+
+
+class Solver:
+    """One device-resident grid (phi, T, theta) and its parameters."""
+
+    def __init__(self, nx: int, ny: int, precision: str = "f32", params: Params | None = None,
+                 **overrides):
+        if precision not in _PREC:
+            raise ValueError(f"precision must be one of {sorted(_PREC)}")
+        p = params if params is not None else default_params()
+        if overrides:
+            p = p.replace(**overrides)
+        self.nx, self.ny = int(nx), int(ny)
+        self.precision = _PREC[precision]
+        self.dtype = np.float64 if self.precision == KG_F64 else np.float32
+        self._h = ctypes.c_void_p()
+        check(lib().kg_create(self.nx, self.ny, self.precision, ctypes.byref(p),
+                              ctypes.byref(self._h)))
+
+    # -- lifecycle --------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            check(lib().kg_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- parameters -------------------------------------------------------
+    @property
+    def params(self) -> Params:
+        p = Params()
+        check(lib().kg_get_params(self._h, ctypes.byref(p)))
+        return p
+
+    @params.setter
+    def params(self, p: Params) -> None:
+        check(lib().kg_set_params(self._h, ctypes.byref(p)))
+
+    def set_params(self, **kw) -> None:
+        self.params = self.params.replace(**kw)
+
+    def set_kernel(self, name: str) -> None:
+        check(lib().kg_set_kernel(self._h, _KERNELS[name]))
+
+    # -- initial states ---------------------------------------------------
+    def reset(self) -> None:
+        check(lib().kg_reset(self._h))
+
+    def reset_disk(self, cx: int, cy: int, r: float) -> None:
+        check(lib().kg_reset_disk(self._h, int(cx), int(cy), float(r)))
+
+    def reset_seeds(self, count: int, r: float, margin: int, seed: int) -> np.ndarray:
+        cxy = (ctypes.c_int * (2 * count))()
+        check(lib().kg_reset_seeds(self._h, int(count), float(r), int(margin), int(seed), cxy))
+        return np.array(list(cxy), dtype=np.int32).reshape(count, 2)
+
+    # -- stepping ---------------------------------------------------------
+    def step(self, n: int = 1) -> None:
+        check(lib().kg_step(self._h, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        ms = ctypes.c_double(0.0)
+        check(lib().kg_step_timed(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self) -> None:
+        check(lib().kg_synchronize(self._h))
+
+    @property
+    def step_count(self) -> int:
+        v = ctypes.c_uint64(0)
+        check(lib().kg_get_step_count(self._h, ctypes.byref(v)))
+        return v.value
+
+    @step_count.setter
+    def step_count(self, v: int) -> None:
+        check(lib().kg_set_step_count(self._h, int(v)))
+
+    # -- fields -----------------------------------------------------------
+    def _get(self, fn) -> np.ndarray:
+        a = np.empty((self.ny, self.nx), self.dtype)
+        check(fn(self._h, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+        return a
+
+    def phase(self) -> np.ndarray:
+        return self._get(lib().kg_get_phase)
+
+    def temperature(self) -> np.ndarray:
+        return self._get(lib().kg_get_temperature)
+
+    def angle(self) -> np.ndarray:
+        return self._get(lib().kg_get_angle)
+
+    def fields(self):
+        return self.phase(), self.temperature(), self.angle()
+
+    def set_fields(self, phi, T, theta=None) -> None:
+        shape = (self.ny, self.nx)
+        arrs = [np.ascontiguousarray(a, self.dtype) for a in (phi, T)]
+        th = None if theta is None else np.ascontiguousarray(theta, self.dtype)
+        for a in arrs + ([th] if th is not None else []):
+            if a.shape != shape:
+                raise ValueError(f"field shape {a.shape} != {shape}")
+        check(lib().kg_set_fields(self._h, arrs[0].ctypes.data_as(ctypes.c_void_p),
+                                  arrs[1].ctypes.data_as(ctypes.c_void_p),
+                                  None if th is None else th.ctypes.data_as(ctypes.c_void_p),
+                                  arrs[0].nbytes))
+
+    def device_fields(self):
+        """(phi, T, theta) device addresses, valid until the next step/reset."""
+        p, t, th = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().kg_device_fields(self._h, ctypes.byref(p), ctypes.byref(t), ctypes.byref(th)))
+        return p.value, t.value, th.value
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        check(lib().kg_get_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    # -- reductions / readback -------------------------------------------
+    def stats(self) -> dict:
+        s = Stats()
+        check(lib().kg_reduce(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def colormap(self, order: int = 0) -> np.ndarray:
+        out = np.empty((self.nx * self.ny, 4), np.float32)
+        check(lib().kg_colormap(self._h, out.ctypes.data_as(ctypes.c_void_p), out.nbytes, order))
+        return out
+
+
+class Kobayashi(Solver):
+    """The reference class's simulation surface (src/Kobayashi.h:32-67) without
+    Win32/D3D: the GUI's 10 steps per frame, reset, object count and colours."""
+
+    STEPS_PER_UPDATE = 10  # src/Kobayashi.cpp:230
+
+    def __init__(self, x: int, y: int, timeStep: float, precision: str = "f32"):
+        super().__init__(x, y, precision, default_params(dt=float(np.float32(timeStep))))
+        self._updateFlag = True
+        self.simFrame = 0
+
+    def iUpdate(self) -> None:
+        self.step(self.STEPS_PER_UPDATE)
+        self.simFrame += 1
+
+    def iResetSimulationState(self) -> None:
+        self.reset()
+        self.simFrame = 0
+
+    def iGetObjectCount(self):
+        return (self.nx, self.ny, 0)
+
+    def iIsUpdated(self) -> bool:
+        return self._updateFlag
+
+    def iUpdateConstantBuffer(self) -> np.ndarray:
+        """Colours of every constant buffer entry, in the reference's order."""
+        return self.colormap(order=1)

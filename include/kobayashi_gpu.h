@@ -1,0 +1,161 @@
+/*
+ * kobayashi_gpu.h -- C ABI of the MI355X (gfx950) Kobayashi phase-field solver.
+ *
+ * Drop-in for the numerical core of jklae/CrystalGrowth's `Kobayashi` class
+ * (src/Kobayashi.h:32-122, src/Kobayashi.cpp:7-249): plain C types, opaque
+ * handles, int status codes.  Every function returns KG_OK (0) or a negative
+ * KG_ERR_* code; kg_last_error() then describes the failure (per thread).
+ *
+ * Fields are nx*ny arrays, x fastest (index i + nx*j, src/Kobayashi.h:91), in
+ * the handle's precision (KG_F32: float, KG_F64: double).  The persistent
+ * state is (phase phi, temperature T, angle theta): theta is only rewritten
+ * where the phase gradient is non-flat (src/Kobayashi.cpp:154-167), so it
+ * carries state across steps.
+ *
+ * Threading: one HIP stream per handle; calls on one handle must not overlap;
+ * distinct handles are independent.  Host buffers are caller-owned.
+ */
+#ifndef KOBAYASHI_GPU_H
+#define KOBAYASHI_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KG_ABI_VERSION 1
+
+/* status codes */
+#define KG_OK 0
+#define KG_ERR_INVALID_ARGUMENT (-1)
+#define KG_ERR_UNSTABLE (-2)      /* explicit-Euler stability bound violated */
+#define KG_ERR_OUT_OF_MEMORY (-3)
+#define KG_ERR_DEVICE (-4)        /* HIP runtime error or no usable GPU */
+#define KG_ERR_SIZE (-5)          /* host buffer size does not match the field */
+
+/* precision of a handle's fields and arithmetic */
+#define KG_F32 0
+#define KG_F64 1
+
+/* boundary maps of every neighbour index (both passes) */
+#define KG_PERIODIC 0             /* reference: i +- 1 wrapped (src/Kobayashi.cpp:133-136) */
+#define KG_ZERO_FLUX 1            /* i +- 1 clamped to the grid */
+
+/* step kernels */
+#define KG_KERNEL_FUSED 0         /* one LDS-tiled launch per step (default) */
+#define KG_KERNEL_TWO_PASS 1      /* pass 1 + pass 2 through global temporaries */
+
+typedef struct kg_params {
+    double dx, dy, dt;            /* ctor: _dx=_dy=0.03, _dt=timeStep (src/Kobayashi.cpp:61-63) */
+    double tau, eps_bar, mu;      /* _parameterInit (src/Kobayashi.cpp:76-78); mu is unused, as in the reference */
+    double K, delta, aniso;       /* latent heat, anisotropy strength, mode number j (:79-81) */
+    double alpha, gamma, teq;     /* m(T) = alpha/pi atan(gamma (Teq - T)) (:82-84, :206) */
+    double theta0;                /* anisotropy offset; 0 reproduces the reference */
+    double noise_a;               /* noise amplitude a; 0 reproduces the reference */
+    int32_t boundary;             /* KG_PERIODIC (reference) or KG_ZERO_FLUX */
+    uint32_t instance;            /* ensemble member id, part of the noise counter */
+    uint64_t seed;                /* Philox4x32-10 key of the noise */
+} kg_params;
+
+typedef struct kg_stats {
+    double sum_phi;               /* FP64 sums over all cells */
+    double sum_T;
+    double sum_T_minus_K_phi;     /* invariant of the scheme up to rounding */
+    double min_phi, max_phi;
+    double solid_fraction;        /* fraction of cells with phi > 1/2 */
+    int64_t nonfinite;            /* cells whose phi or T is NaN or infinite */
+    int64_t cells;
+} kg_stats;
+
+typedef struct kg_solver_s *kg_handle;
+
+/* ---- library ------------------------------------------------------------ */
+int kg_abi_version(void);
+const char *kg_last_error(void);
+/* Number of visible GPUs (0 without a GPU; never an error). */
+int kg_device_count(int *count);
+
+/* ---- parameters (host only; no GPU needed) ------------------------------ */
+/* Reference defaults: constructor + _parameterInit (src/Kobayashi.cpp:61-63,
+ * :73-96) with dt = 1e-4 as src/main.cpp:16 passes it. */
+int kg_default_params(kg_params *out);
+/* Checks sizes (nx, ny >= 3), finite positive dx, dy, dt, tau, eps_bar, and
+ * the explicit-Euler bound dt/dx^2 <= 3/8 of the 9-point Laplacian (also for
+ * the phase equation's diffusion eps_bar^2 (1+|delta|)^2 / tau). */
+int kg_validate(int nx, int ny, const kg_params *p);
+/* Bit k of *mask set = parameter k outside the reference GUI's slider range
+ * (src/Kobayashi.cpp:17-58; order tau, eps_bar, mu, K, delta, aniso, alpha,
+ * gamma, teq).  Informational: out-of-range values are still accepted. */
+int kg_reference_range_mask(const kg_params *p, uint32_t *mask);
+/* Philox4x32-10 block (host evaluation of the device noise source). */
+int kg_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- lifecycle ------------------------------------------------------------ */
+/* Replaces Kobayashi::Kobayashi(int x, int y, float timeStep)
+ * (src/Kobayashi.h:35, src/Kobayashi.cpp:7-67): allocates device fields and
+ * applies the reference initial state (kg_reset). */
+int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out);
+/* Replaces Kobayashi::~Kobayashi (src/Kobayashi.cpp:69-71). */
+int kg_destroy(kg_handle h);
+
+/* ---- initial states --------------------------------------------------------- */
+/* Replaces _vectorInit + _createNucleus(nx/2, ny/2) (src/Kobayashi.cpp:98-123),
+ * as iResetSimulationState calls it (:241-249): all fields 0, phi = 1 on the
+ * 5-cell plus at (nx/2, ny/2); the step counter returns to 0. */
+int kg_reset(kg_handle h);
+/* All fields 0, phi = 1 where (i-cx)^2 + (j-cy)^2 <= r^2; counter to 0. */
+int kg_reset_disk(kg_handle h, int cx, int cy, double r);
+/* All fields 0, `count` disks of radius r at Philox-drawn centres at least
+ * `margin` cells from every edge (centres in cxy[2*count] if non-NULL). */
+int kg_reset_seeds(kg_handle h, int count, double r, int margin, uint64_t seed, int *cxy);
+
+/* ---- parameters of a handle ------------------------------------------------- */
+/* Replaces the slider writes (src/Kobayashi.cpp:587-611) and RESET's
+ * _parameterInit (:514).  Unlike the GUI it does not reset the fields. */
+int kg_set_params(kg_handle h, const kg_params *p);
+int kg_get_params(kg_handle h, kg_params *p);
+int kg_get_size(kg_handle h, int *nx, int *ny, int *precision);
+
+/* ---- stepping ------------------------------------------------------------------ */
+/* nsteps explicit-Euler steps: _computeGradientLaplacian + _evolution
+ * (src/Kobayashi.cpp:125-221) per step; Kobayashi::iUpdate is kg_step(h, 10)
+ * (src/Kobayashi.cpp:227-239).  Asynchronous on the handle's stream. */
+int kg_step(kg_handle h, int nsteps);
+/* kg_step bracketed by HIP events on the handle's stream; *ms = elapsed
+ * milliseconds of the nsteps launches (waits for completion). */
+int kg_step_timed(kg_handle h, int nsteps, double *ms);
+int kg_synchronize(kg_handle h);
+/* Steps taken since the last reset (the noise counter of the next step). */
+int kg_get_step_count(kg_handle h, uint64_t *steps);
+int kg_set_step_count(kg_handle h, uint64_t steps);
+int kg_set_kernel(kg_handle h, int kernel);
+
+/* ---- fields --------------------------------------------------------------------- */
+/* Copies of _phi, _t, _angl (src/Kobayashi.h:107-115); bytes must equal
+ * nx*ny*sizeof(element).  Synchronous. */
+int kg_get_phase(kg_handle h, void *dst, size_t bytes);
+int kg_get_temperature(kg_handle h, void *dst, size_t bytes);
+int kg_get_angle(kg_handle h, void *dst, size_t bytes);
+/* Replaces the state; theta may be NULL (then 0).  Does not touch the counter. */
+int kg_set_fields(kg_handle h, const void *phi, const void *T, const void *theta, size_t bytes);
+/* Device addresses of the current fields (valid until the next kg_step or
+ * kg_reset*) and the handle's stream (hipStream_t). */
+int kg_device_fields(kg_handle h, void **phi, void **T, void **theta);
+int kg_get_stream(kg_handle h, void **stream);
+
+/* ---- reductions and readback --------------------------------------------------------- */
+/* FP64 reductions over the grid, deterministic (fixed two-level order). */
+int kg_reduce(kg_handle h, kg_stats *out);
+/* phi -> RGBA colour of iUpdateConstantBuffer's three-segment ramp
+ * (src/Kobayashi.cpp:309-345), 4 floats per cell with alpha 1.  order 0: x
+ * fastest; order 1: the reference's constant-buffer order, entry i showing
+ * _phi[_INDEX(i / n, i % n)] (square grids only). */
+int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KOBAYASHI_GPU_H */

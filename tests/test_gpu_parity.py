@@ -1,0 +1,292 @@
+"""HIP kernels vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Tolerances (written here, per the precision of the path):
+  FP64 kernel vs FP64 oracle: max |diff| <= 1e-10 on phi, T (any horizon used
+      here); theta equal to 1e-10 modulo 2 pi.  Only libm-vs-device rounding
+      (~1e-16) separates them.
+  FP32 kernel vs FP32 oracle, one step: max |diff| <= 2e-6 (a few ulp of O(1)
+      values; device atanf/sincosf, reciprocal multiplies and FMA differ from
+      glibc's float functions and the reference's divisions by ulps).
+  FP32 kernel vs FP64 oracle at 100 steps on the disk-seeded configurations:
+      relative L2 <= 1e-5 on phi and on T (the north-star accuracy bar).
+The plus-shaped reference nucleus is compared per cell only for 4 steps:
+its centre has an exactly zero gradient, and from step 5 rounding noise
+decides whether theta fires there (src/Kobayashi.cpp:154-167), in any two
+arithmetic paths alike; longer horizons are compared on morphology.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FLOAT, DOUBLE = np.float32, np.float64
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def angdiff(a, b):
+    d = np.mod(a.astype(float) - b.astype(float) + np.pi, 2 * np.pi) - np.pi
+    return np.abs(d)
+
+
+def well_conditioned(phi):
+    """Cells whose phase differences across the stencil are >= 1e-3: there
+    theta = atan(gy/gx) is fixed to rounding; in the precursor fringe (phi ~
+    1e-20) the ratio of two tiny differences is not, and theta only
+    multiplies those tiny gradients."""
+    gx = np.abs(np.roll(phi, -1, 1) - np.roll(phi, 1, 1))
+    gy = np.abs(np.roll(phi, -1, 0) - np.roll(phi, 1, 0))
+    return np.maximum(gx, gy) >= 1e-3
+
+
+@pytest.fixture(scope="module")
+def kb():
+    import crystalgrowth_amd as kb
+    if kb.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return kb
+
+
+def gpu_params(kb, op):
+    """kg_params with the oracle's values."""
+    return kb.Params(**op.as_dict())
+
+
+def run_gpu(kb, state, op, nsteps, precision, kernel="fused", step0=0):
+    ny, nx = state.shape
+    with kb.Solver(nx, ny, precision, gpu_params(kb, op)) as s:
+        s.set_kernel(kernel)
+        dt = np.float64 if precision == "f64" else np.float32
+        s.set_fields(state.phi.astype(dt), state.T.astype(dt), state.theta.astype(dt))
+        s.step_count = step0
+        s.step(nsteps)
+        return s.fields()
+
+
+CASES = {
+    # name: (nx, ny, params, init)
+    "ref_plus_periodic": (64, 48, dict(), "plus"),
+    "c0_disk_zero_flux_noise": (96, 80, dict(delta=0.02, aniso=4.0, theta0=0.2, noise_a=0.01, boundary=1, seed=7), "disk"),
+    "c1_disk_zero_flux_noise": (130, 70, dict(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1, seed=3), "disk"),
+    "odd_sizes_periodic_noise": (37, 23, dict(delta=0.04, aniso=5.0, noise_a=0.02, seed=11, instance=2), "disk"),
+    "tiny_zero_flux": (5, 4, dict(boundary=1), "random"),
+    "random_periodic": (70, 33, dict(aniso=4.0, theta0=0.7), "random"),
+}
+
+
+def make_state(O, name, dtype):
+    nx, ny, kw, init = CASES[name]
+    p = O.default_params(**kw)
+    if init == "plus":
+        s = O.init_plus(nx, ny)
+    elif init == "disk":
+        s = O.init_disk(nx, ny, nx // 2 - 3, ny // 2 + 2, 6.0)
+    else:
+        rng = np.random.default_rng(nx * 1000 + ny)
+        s = O.State(rng.random((ny, nx)), rng.random((ny, nx)) * 0.6, rng.uniform(-1, 7, (ny, nx)))
+    if dtype == np.float32:
+        # same float-rounded parameters on both sides of an FP32 comparison
+        p = O.Params(**{k: (float(np.float32(v)) if isinstance(v, float) else v)
+                        for k, v in p.as_dict().items()})
+    return s.astype(dtype), p
+
+
+@pytest.mark.parametrize("kernel", ["fused", "two_pass"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp64_kernel_matches_fp64_oracle(kb, O, name, kernel):
+    s, p = make_state(O, name, DOUBLE)
+    nsteps = 4 if CASES[name][3] == "plus" else 40
+    ref = O.run(s, p, nsteps)
+    phi, T, th = run_gpu(kb, s, p, nsteps, "f64", kernel)
+    assert np.abs(phi - ref.phi).max() <= 1e-10
+    assert np.abs(T - ref.T).max() <= 1e-10
+    d = angdiff(th, ref.theta)
+    assert d[well_conditioned(ref.phi)].max(initial=0) <= 1e-10
+    assert d.max() <= 1e-6
+
+
+@pytest.mark.parametrize("kernel", ["fused", "two_pass"])
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_fp32_kernel_one_step_matches_fp32_oracle(kb, O, name, kernel):
+    s, p = make_state(O, name, FLOAT)
+    ref = O.step(s, p)
+    phi, T, th = run_gpu(kb, s, p, 1, "f32", kernel)
+    assert np.abs(phi.astype(float) - ref.phi).max() <= 2e-6
+    assert np.abs(T.astype(float) - ref.T).max() <= 2e-6
+    # theta: same branch decisions; values to float rounding (mod 2 pi)
+    assert angdiff(th, ref.theta).max() <= 2e-5
+
+
+def test_fp32_plus_nucleus_four_steps(kb, O):
+    s, p = make_state(O, "ref_plus_periodic", FLOAT)
+    ref = O.run(s, p, 4)
+    phi, T, _ = run_gpu(kb, s, p, 4, "f32")
+    assert np.abs(phi - ref.phi).max() <= 4e-6
+    assert np.abs(T - ref.T).max() <= 4e-6
+
+
+@pytest.mark.parametrize("cfg", ["c0", "c1"])
+def test_fp32_within_1e5_of_fp64_at_100_steps(kb, O, cfg):
+    """North-star accuracy: FP32 step within 1e-5 relative (L2) of FP64 after
+    100 steps, on the configurations' own setup (disk seeds, zero-flux, noise)."""
+    if cfg == "c0":
+        kw = dict(delta=0.02, aniso=4.0, theta0=0.2, noise_a=0.01, boundary=1)
+    else:
+        kw = dict(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1)
+    p = O.default_params(**kw)
+    s = O.init_disk(160, 144, 80, 72, 5.0)
+    ref64 = O.run(s, p, 100)
+    ref32 = O.run(s.astype(FLOAT), p, 100)
+    phi, T, _ = run_gpu(kb, s.astype(FLOAT), p, 100, "f32")
+    assert rel(phi, ref64.phi) <= 1e-5 and rel(T, ref64.T) <= 1e-5
+    assert rel(phi, ref32.phi) <= 1e-5 and rel(T, ref32.T) <= 1e-5
+    # the GPU's FP32 error is of the same order as the reference arithmetic's
+    assert rel(phi, ref64.phi) <= 10 * max(rel(ref32.phi, ref64.phi), 1e-7)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_fused_equals_two_pass(kb, O, precision):
+    s, p = make_state(O, "c1_disk_zero_flux_noise", FLOAT if precision == "f32" else DOUBLE)
+    a = run_gpu(kb, s, p, 30, precision, "fused")
+    b = run_gpu(kb, s, p, 30, precision, "two_pass")
+    tol = 1e-5 if precision == "f32" else 1e-11
+    for x, y in zip(a[:2], b[:2]):
+        assert np.abs(x.astype(float) - y).max() <= tol
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_translation_invariance_bitwise(kb, O, precision):
+    """Periodic, no noise: shifting the state shifts the result bit for bit,
+    whatever tile (flat fast path or full arithmetic) a cell lands in."""
+    dt = FLOAT if precision == "f32" else DOUBLE
+    nx, ny = 200, 90
+    p = O.default_params(delta=0.04)
+    s = O.init_disk(nx, ny, 60, 40, 7.0).astype(dt)
+    a = run_gpu(kb, s, p, 25, precision)
+    sh = (17, 45)
+    r = O.State(*(np.roll(x, sh, (0, 1)) for x in (s.phi, s.T, s.theta)))
+    b = run_gpu(kb, r, p, 25, precision)
+    for x, y in zip(a, b):
+        assert np.array_equal(np.roll(x, sh, (0, 1)), y)
+
+
+def test_noise_field_matches_oracle(kb, O):
+    """Uniform phi, T: one FP64 step differs from the noiseless one exactly by
+    p(1-p) a chi(i, j, step) dt / tau, with chi from the shared counter RNG."""
+    nx, ny = 67, 41
+    p = O.default_params(noise_a=0.02, seed=0x1234ABCD5678, instance=5)
+    s = O.zeros(nx, ny, DOUBLE)
+    s.phi[:] = 0.25
+    s.T[:] = 0.5
+    phi_n, _, _ = run_gpu(kb, s, p, 1, "f64", step0=123)
+    phi_0, _, _ = run_gpu(kb, s, p.replace(noise_a=0.0), 1, "f64", step0=123)
+    chi = np.array([[O.noise(p, i, j, 123) for i in range(nx)] for j in range(ny)])
+    m = p.alpha / np.pi * np.arctan(p.gamma * (p.teq - 0.5))
+    expect = 0.25 * 0.75 * (0.25 - 0.5 + m + p.noise_a * chi) * p.dt / p.tau + 0.25
+    np.testing.assert_allclose(phi_n, expect, rtol=0, atol=1e-15)
+    assert not np.allclose(phi_n, phi_0)
+
+
+def test_initial_states_match_oracle(kb, O):
+    with kb.Solver(250, 250) as s:
+        phi, T, th = s.fields()
+        ref = O.init_plus(250, 250)
+        assert np.array_equal(phi, ref.phi) and not T.any() and not th.any()
+        s.reset_disk(40, 200, 5.0)
+        assert np.array_equal(s.phase(), O.init_disk(250, 250, 40, 200, 5.0).phi)
+        cxy = s.reset_seeds(9, 5.0, 8, 4242)
+        ref, rcxy = O.init_seeds(250, 250, 9, 5.0, 8, 4242)
+        assert np.array_equal(cxy, rcxy)
+        assert np.array_equal(s.phase(), ref.phi)
+        assert s.step_count == 0
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_reductions(kb, O, precision):
+    p = O.default_params(delta=0.04, noise_a=0.01, boundary=1)
+    with kb.Solver(300, 200, precision, gpu_params(kb, p)) as s:
+        s.reset_disk(150, 100, 9.0)
+        inv0 = s.stats()["sum_T_minus_K_phi"]
+        s.step(200)
+        st = s.stats()
+        phi, T, _ = s.fields()
+        K = float(np.float32(p.K)) if precision == "f32" else p.K
+        assert st["cells"] == 300 * 200 and st["nonfinite"] == 0
+        np.testing.assert_allclose(st["sum_phi"], phi.astype(float).sum(), rtol=1e-12)
+        np.testing.assert_allclose(st["sum_T"], T.astype(float).sum(), rtol=1e-12)
+        assert st["max_phi"] == phi.max() and st["min_phi"] == phi.min()
+        assert st["solid_fraction"] == (phi > 0.5).mean()
+        # sum(T - K phi) is invariant up to rounding
+        tol = 1e-4 if precision == "f32" else 1e-9
+        assert abs(st["sum_T_minus_K_phi"] - inv0) <= tol * abs(inv0)
+        np.testing.assert_allclose(st["sum_T_minus_K_phi"], (T.astype(float) - K * phi.astype(float)).sum(), rtol=1e-9)
+
+
+def test_nonfinite_is_reported(kb):
+    with kb.Solver(40, 40, "f64") as s:
+        phi, T, th = s.fields()
+        T[3, 4] = np.nan
+        phi[10, 11] = np.inf
+        s.set_fields(phi, T, th)
+        assert s.stats()["nonfinite"] == 2
+
+
+def test_colormap_ramp(kb):
+    """iUpdateConstantBuffer's ramp (src/Kobayashi.cpp:318-344) and its
+    transposed entry order."""
+    n = 16
+    phi = np.linspace(0, 1, n * n).reshape(n, n).astype(np.float32)
+    with kb.Solver(n, n) as s:
+        s.set_fields(phi, np.zeros_like(phi))
+        c0 = s.colormap(0)
+        c1 = s.colormap(1)
+    c = [np.array(v, np.float32) for v in ((0, 0, 0), (0.2505490, 0.5, 0.9882353),
+                                           (0.3607843, 1.0, 0.9882353), (0.9005490, 1.0, 0.9882353))]
+    flat = phi.ravel()
+    for k in (0, 37, 229, 230, 254, 255):
+        v = flat[k]
+        if v <= np.float32(0.9):
+            r = v * (np.float32(1) / np.float32(0.9)); lo, hi = c[0], c[1]
+        elif v <= np.float32(0.99):
+            r = (v - np.float32(0.9)) * (np.float32(1) / (np.float32(0.99) - np.float32(0.9))); lo, hi = c[1], c[2]
+        else:
+            r = (v - np.float32(0.99)) * (np.float32(1) / (np.float32(1.0) - np.float32(0.99))); lo, hi = c[2], c[3]
+        np.testing.assert_allclose(c0[k, :3], lo * (1 - r) + hi * r, rtol=0, atol=2e-7)
+        assert c0[k, 3] == 1.0
+    # entry i of the reference buffer shows _phi[(i // n) + n * (i % n)]
+    i = np.arange(n * n)
+    np.testing.assert_array_equal(c1, c0[(i // n) + n * (i % n)])
+
+
+def test_kobayashi_class_surface(kb, O):
+    """The reference's default run (src/main.cpp:14-18): Kobayashi(250, 250,
+    1e-4f), 10 steps per iUpdate."""
+    k = kb.Kobayashi(250, 250, 0.0001)
+    assert k.iGetObjectCount() == (250, 250, 0) and k.iIsUpdated()
+    k.iUpdate()
+    assert k.step_count == 10 and k.simFrame == 1
+    ref = O.run(O.init_plus(250, 250, FLOAT), O.default_params(), 4)
+    k.iResetSimulationState()
+    k.step(4)
+    assert np.abs(k.phase() - ref.phi).max() <= 4e-6
+    assert k.iUpdateConstantBuffer().shape == (250 * 250, 4)
+    k.close()
+
+
+def test_set_get_and_step_counter(kb):
+    with kb.Solver(33, 17, "f64") as s:
+        rng = np.random.default_rng(0)
+        a, b, c = rng.random((17, 33)), rng.random((17, 33)), rng.random((17, 33))
+        s.set_fields(a, b, c)
+        x, y, z = s.fields()
+        assert np.array_equal(a, x) and np.array_equal(b, y) and np.array_equal(c, z)
+        s.step(3)
+        assert s.step_count == 3
+        s.step_count = 100
+        assert s.step_count == 100
+        with pytest.raises(ValueError):
+            s.set_fields(a[:5], b)
+        with pytest.raises(kb.KGError):
+            s.set_params(dt=1.0)
