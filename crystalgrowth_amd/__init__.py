@@ -6,7 +6,8 @@ include/kobayashi_gpu.h (libkobayashi_gpu.so, built in-tree by
 `Solver` (general handle) and `Kobayashi` (the reference class's surface).
 """
 from ._lib import KGError, Params, Stats  # noqa: F401
-from .kobayashi import Kobayashi, Solver, default_params, device_count, philox4x32_10, validate  # noqa: F401
+from .kobayashi import (Batch, Kobayashi, Solver, default_params, device_count,  # noqa: F401
+                        philox4x32_10, validate)
 
-__all__ = ["Kobayashi", "Solver", "Params", "Stats", "KGError", "default_params", "device_count",
-           "philox4x32_10", "validate"]
+__all__ = ["Batch", "Kobayashi", "Solver", "Params", "Stats", "KGError", "default_params",
+           "device_count", "philox4x32_10", "validate"]

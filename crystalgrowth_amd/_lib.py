@@ -22,6 +22,7 @@ KG_ERR_SIZE = -5
 KG_F32, KG_F64 = 0, 1
 KG_PERIODIC, KG_ZERO_FLUX = 0, 1
 KG_KERNEL_FUSED, KG_KERNEL_TWO_PASS = 0, 1
+KG_BATCH_LAUNCH, KG_BATCH_GRAPH = 0, 1
 
 
 class KGError(RuntimeError):
@@ -51,6 +52,17 @@ class Params(ctypes.Structure):
                 raise AttributeError(f"kg_params has no field {k!r}")
             setattr(q, k, v)
         return q
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Grads(ctypes.Structure):
+    """kg_grads (include/kobayashi_gpu.h): loss and dL/d(parameter)."""
+
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "loss", "d_tau", "d_eps_bar", "d_K", "d_delta", "d_theta0", "d_alpha", "d_gamma",
+        "d_teq", "d_noise_a")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -108,6 +120,24 @@ SIGNATURES = {
     "kg_get_stream": [H, ctypes.POINTER(VP)],
     "kg_reduce": [H, ctypes.POINTER(Stats)],
     "kg_colormap": [H, VP, SZ, I],
+    "kg_adjoint": [H, I, I, VP, SZ, ctypes.POINTER(Grads), VP, VP],
+    "kg_batch_create": [I, PI, PI, I, P, ctypes.POINTER(H)],
+    "kg_batch_destroy": [H],
+    "kg_batch_count": [H, PI],
+    "kg_batch_reset_disk": [H, I, I, I, ctypes.c_double],
+    "kg_batch_reset_seeds": [H, I, I, ctypes.c_double, I, U64],
+    "kg_batch_set_fields": [H, I, VP, VP, VP, SZ],
+    "kg_batch_get_phase": [H, I, VP, SZ],
+    "kg_batch_get_temperature": [H, I, VP, SZ],
+    "kg_batch_get_angle": [H, I, VP, SZ],
+    "kg_batch_reduce": [H, I, ctypes.POINTER(Stats)],
+    "kg_batch_set_mode": [H, I, I],
+    "kg_batch_get_mode": [H, PI, PI],
+    "kg_batch_autotune": [H, I, PI, PI],
+    "kg_batch_step": [H, I],
+    "kg_batch_step_timed": [H, I, ctypes.POINTER(ctypes.c_double)],
+    "kg_batch_synchronize": [H],
+    "kg_batch_get_step_count": [H, ctypes.POINTER(U64)],
 }
 
 

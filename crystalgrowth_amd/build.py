@@ -23,7 +23,7 @@ BIN = os.path.join(PKG, "bin")
 HEADLESS = os.path.join(BIN, "kobayashi_headless")
 
 ARCH = "gfx950"
-HIP_SOURCES = ["kg_step.hip", "kg_api.hip"]
+HIP_SOURCES = ["kg_step.hip", "kg_api.hip", "kg_batch.hip", "kg_adjoint.hip"]
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 (v_pk_*_f32) stays off: hand-written HIP must not rely on it

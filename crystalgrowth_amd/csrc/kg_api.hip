@@ -14,12 +14,16 @@
 #include <vector>
 
 #include "../../include/kobayashi_gpu.h"
+#include "kg_host.h"
 #include "kg_internal.h"
 #include "kg_philox.h"
 
 namespace {
-
 thread_local std::string g_last_error;
+constexpr long long kMaxCells = 1ll << 32;  // 4 Gi cells (64 GiB per FP32 field pair)
+}  // namespace
+
+namespace kg {
 
 int fail(int code, const std::string &msg)
 {
@@ -32,39 +36,24 @@ int hip_fail(hipError_t e, const char *what)
     return fail(KG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define KG_HIP(call)                               \
-    do {                                           \
-        hipError_t e_ = (call);                    \
-        if (e_ != hipSuccess)                      \
-            return hip_fail(e_, #call);            \
-    } while (0)
+void seed_centres(int nx, int ny, int count, int margin, uint64_t seed, int *cxy)
+{
+    const uint32_t spanx = (uint32_t)(nx - 2 * margin), spany = (uint32_t)(ny - 2 * margin);
+    for (int k = 0; k < count; ++k) {
+        // counter word 3 = 0xFFFFFFFF keeps these draws apart from the noise
+        // counters (word 3 = ensemble member id)
+        const u32x4 v = philox4x32_10(u32x4{(uint32_t)k, 0u, 0u, 0xFFFFFFFFu}, (uint32_t)seed,
+                                      (uint32_t)(seed >> 32));
+        cxy[2 * k] = margin + (int)(v.x % spanx);
+        cxy[2 * k + 1] = margin + (int)(v.y % spany);
+    }
+}
 
-constexpr long long kMaxCells = 1ll << 32;  // 4 Gi cells (64 GiB per FP32 field pair)
+}  // namespace kg
 
-}  // namespace
+using kg::fail;
+using kg::hip_fail;
 
-struct kg_solver_s {
-    int nx = 0, ny = 0, precision = KG_F32;
-    kg_params params{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    void *phi[2] = {nullptr, nullptr};
-    void *T[2] = {nullptr, nullptr};
-    void *theta = nullptr;
-    void *temps = nullptr;  // 6 fields, two-pass kernel only
-    double *partials = nullptr;
-    kg::ReduceOut *red = nullptr;
-    int *disks = nullptr;
-    int disks_cap = 0;
-    int cur = 0;
-    uint64_t steps = 0;
-    int kernel = KG_KERNEL_FUSED;
-
-    size_t cells() const { return (size_t)nx * (size_t)ny; }
-    size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
-    size_t field_bytes() const { return cells() * esize(); }
-};
 
 namespace {
 
@@ -80,36 +69,12 @@ int check_handle(kg_handle h)
 template <typename R>
 kg::StepArgs<R> make_args(const kg_solver_s &h)
 {
-    const kg_params &p = h.params;
-    kg::StepArgs<R> a{};
+    kg::StepArgs<R> a = kg::make_args<R>(h.params, h.nx, h.ny);
     a.phi_in = static_cast<const R *>(h.phi[h.cur]);
     a.T_in = static_cast<const R *>(h.T[h.cur]);
     a.phi_out = static_cast<R *>(h.phi[h.cur ^ 1]);
     a.T_out = static_cast<R *>(h.T[h.cur ^ 1]);
     a.theta = static_cast<R *>(h.theta);
-    a.nx = h.nx;
-    a.ny = h.ny;
-    a.boundary = p.boundary;
-    // scalar combinations in R, in the reference's association order
-    const R dx = (R)p.dx, dy = (R)p.dy, dt = (R)p.dt, tau = (R)p.tau;
-    a.inv_dx = (R)1 / dx;
-    a.inv_dy = (R)1 / dy;
-    a.inv_3dx2 = (R)1 / ((R)3 * dx * dx);
-    a.dt = dt;
-    a.dt_tau = dt / tau;
-    a.eps_bar = (R)p.eps_bar;
-    a.delta = (R)p.delta;
-    a.aniso = (R)p.aniso;
-    a.theta0 = (R)p.theta0;
-    a.neg_ebar_j_delta = -(R)p.eps_bar * (R)p.aniso * (R)p.delta;
-    a.alpha_pi = (R)p.alpha / (sizeof(R) == 4 ? (R)3.14159274f : (R)3.14159265358979323846);
-    a.gamma = (R)p.gamma;
-    a.teq = (R)p.teq;
-    a.K = (R)p.K;
-    a.noise_a = (R)p.noise_a;
-    a.instance = p.instance;
-    a.key0 = (uint32_t)p.seed;
-    a.key1 = (uint32_t)(p.seed >> 32);
     return a;
 }
 
@@ -187,6 +152,8 @@ void free_all(kg_solver_s *h)
     (void)hipSetDevice(h->device);
     if (h->stream)
         (void)hipStreamSynchronize(h->stream);
+    kg::free_adjoint(h->adj);
+    h->adj = nullptr;
     void *bufs[] = {h->phi[0], h->phi[1], h->T[0], h->T[1], h->theta, h->temps, h->partials,
                     h->red, h->disks};
     for (void *b : bufs)
@@ -401,16 +368,8 @@ int kg_reset_seeds(kg_handle h, int count, double r, int margin, uint64_t seed, 
         return fail(KG_ERR_INVALID_ARGUMENT, "radius must be finite and >= 0");
     if (margin < 0 || 2 * margin >= h->nx || 2 * margin >= h->ny)
         return fail(KG_ERR_INVALID_ARGUMENT, "margin leaves no room for centres");
-    const uint32_t spanx = (uint32_t)(h->nx - 2 * margin), spany = (uint32_t)(h->ny - 2 * margin);
     std::vector<int> d(2 * (size_t)count);
-    for (int k = 0; k < count; ++k) {
-        // counter word 3 = 0xFFFFFFFF keeps these draws apart from the noise
-        // counters (word 3 = ensemble member id)
-        const kg::u32x4 v = kg::philox4x32_10(kg::u32x4{(uint32_t)k, 0u, 0u, 0xFFFFFFFFu},
-                                              (uint32_t)seed, (uint32_t)(seed >> 32));
-        d[2 * k] = margin + (int)(v.x % spanx);
-        d[2 * k + 1] = margin + (int)(v.y % spany);
-    }
+    kg::seed_centres(h->nx, h->ny, count, margin, seed, d.data());
     if (cxy)
         std::memcpy(cxy, d.data(), d.size() * sizeof(int));
     return init_fields(*h, 1, 0, 0, d, r * r);

@@ -1,0 +1,624 @@
+// kg_batch.hip -- batched ensemble of independent Kobayashi grids (C ABI).
+//
+// All members live in one allocation per field (member offsets rounded up to
+// 256 bytes, so interior tiles keep their 16-byte vector loads).  A device
+// table holds each member's StepArgs; a tile table lists (member, x0, y0) for
+// every 64 x 16 tile of every member in member order, so the XCD-aware block
+// order hands each XCD a contiguous run of members.  One launch steps every
+// tile of every member; KG_BATCH_GRAPH replays a captured hipGraph of an even
+// number of launches followed by a counter kernel, with the noise counter
+// kept on the device so replays draw fresh noise.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kobayashi_gpu.h"
+#include "kg_host.h"
+#include "kg_internal.h"
+
+using kg::fail;
+using kg::hip_fail;
+
+struct kg_batch_s {
+    int count = 0, precision = KG_F32;
+    std::vector<int> nx, ny;
+    std::vector<kg_params> params;
+    std::vector<size_t> off;   // element offset of each member in the field buffers
+    size_t total = 0;          // elements per field buffer
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *phi[2] = {nullptr, nullptr};
+    void *T[2] = {nullptr, nullptr};
+    void *theta = nullptr;
+    void *members = nullptr;   // StepArgs<R>[count]
+    int4 *tiles = nullptr;
+    int ntiles = 0;
+    uint32_t *step_base = nullptr;
+    double *partials = nullptr;
+    kg::ReduceOut *red = nullptr;
+    int *disks = nullptr;
+    int disks_cap = 0;
+    uint64_t steps = 0;        // steps since the last reset of all members
+    uint64_t base_host = 0;    // value of *step_base
+    uint32_t base_tmp = 0;     // host source of step_base uploads
+    int mode = KG_BATCH_LAUNCH, graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
+    size_t cells(int m) const { return (size_t)nx[m] * (size_t)ny[m]; }
+    int cur() const { return (int)(steps & 1u); }
+    char *field(void *base, int m) const { return static_cast<char *>(base) + off[m] * esize(); }
+};
+
+namespace {
+
+int check_batch(kg_batch b, int member, bool allow_all)
+{
+    if (!b)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null batch");
+    if (member >= b->count || (member < 0 && !allow_all))
+        return fail(KG_ERR_INVALID_ARGUMENT, "member index out of range");
+    return KG_OK;
+}
+
+void drop_graph(kg_batch_s *b)
+{
+    if (b->exec)
+        (void)hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void free_batch(kg_batch_s *b)
+{
+    if (!b)
+        return;
+    (void)hipSetDevice(b->device);
+    if (b->stream)
+        (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    void *bufs[] = {b->phi[0], b->phi[1], b->T[0], b->T[1], b->theta, b->members,
+                    b->tiles, b->step_base, b->partials, b->red, b->disks};
+    for (void *p : bufs)
+        if (p)
+            (void)hipFree(p);
+    if (b->ev0)
+        (void)hipEventDestroy(b->ev0);
+    if (b->ev1)
+        (void)hipEventDestroy(b->ev1);
+    if (b->stream)
+        (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+template <typename R>
+std::vector<kg::StepArgs<R>> member_args(const kg_batch_s &b)
+{
+    std::vector<kg::StepArgs<R>> v(b.count);
+    for (int m = 0; m < b.count; ++m) {
+        kg::StepArgs<R> a = kg::make_args<R>(b.params[m], b.nx[m], b.ny[m]);
+        a.phi_in = reinterpret_cast<const R *>(b.field(b.phi[0], m));
+        a.T_in = reinterpret_cast<const R *>(b.field(b.T[0], m));
+        a.phi_out = reinterpret_cast<R *>(b.field(b.phi[1], m));
+        a.T_out = reinterpret_cast<R *>(b.field(b.T[1], m));
+        a.theta = reinterpret_cast<R *>(b.field(b.theta, m));
+        v[m] = a;
+    }
+    return v;
+}
+
+int launch_steps(kg_batch_s &b, int n)
+{
+    for (int s = 0; s < n; ++s) {
+        const uint32_t add = (uint32_t)(b.steps - b.base_host);
+        hipError_t e = b.precision == KG_F64
+                           ? kg::launch_step_batch<double>(
+                                 static_cast<const kg::StepArgs<double> *>(b.members), b.tiles,
+                                 b.ntiles, b.step_base, add, b.stream)
+                           : kg::launch_step_batch<float>(
+                                 static_cast<const kg::StepArgs<float> *>(b.members), b.tiles,
+                                 b.ntiles, b.step_base, add, b.stream);
+        if (e != hipSuccess)
+            return hip_fail(e, "batch step launch");
+        b.steps += 1;
+    }
+    return KG_OK;
+}
+
+int capture_graph(kg_batch_s &b)
+{
+    drop_graph(&b);
+    KG_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < b.graph_steps && e == hipSuccess; ++s)
+        e = b.precision == KG_F64
+                ? kg::launch_step_batch<double>(static_cast<const kg::StepArgs<double> *>(b.members),
+                                                b.tiles, b.ntiles, b.step_base, (uint32_t)s, b.stream)
+                : kg::launch_step_batch<float>(static_cast<const kg::StepArgs<float> *>(b.members),
+                                               b.tiles, b.ntiles, b.step_base, (uint32_t)s, b.stream);
+    if (e == hipSuccess)
+        e = kg::launch_advance(b.step_base, (uint32_t)b.graph_steps, b.stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b.stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return hip_fail(e != hipSuccess ? e : e2, "graph capture");
+    }
+    b.graph = g;
+    KG_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    return KG_OK;
+}
+
+int step_impl(kg_batch_s &b, int nsteps)
+{
+    if (b.mode == KG_BATCH_GRAPH && nsteps >= b.graph_steps) {
+        if (!b.exec)
+            if (int rc = capture_graph(b))
+                return rc;
+        if (b.base_host != b.steps) {
+            b.base_tmp = (uint32_t)b.steps;
+            KG_HIP(hipMemcpyAsync(b.step_base, &b.base_tmp, sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, b.stream));
+            KG_HIP(hipStreamSynchronize(b.stream));
+            b.base_host = b.steps;
+        }
+        while (nsteps >= b.graph_steps) {
+            KG_HIP(hipGraphLaunch(b.exec, b.stream));
+            b.steps += (uint64_t)b.graph_steps;
+            b.base_host += (uint64_t)b.graph_steps;
+            nsteps -= b.graph_steps;
+        }
+    }
+    return launch_steps(b, nsteps);
+}
+
+int init_member(kg_batch_s &b, int m, int kind, int x, int y, const std::vector<int> &disks,
+                double r2)
+{
+    const int nd = (int)disks.size() / 2;
+    if (nd > b.disks_cap) {
+        if (b.disks)
+            KG_HIP(hipFree(b.disks));
+        b.disks = nullptr;
+        KG_HIP(hipMalloc(&b.disks, disks.size() * sizeof(int)));
+        b.disks_cap = nd;
+    }
+    if (nd > 0) {
+        KG_HIP(hipMemcpyAsync(b.disks, disks.data(), disks.size() * sizeof(int),
+                              hipMemcpyHostToDevice, b.stream));
+    }
+    const int c = b.cur();
+    hipError_t e;
+    if (b.precision == KG_F64)
+        e = kg::launch_init<double>(reinterpret_cast<double *>(b.field(b.phi[c], m)),
+                                    reinterpret_cast<double *>(b.field(b.T[c], m)),
+                                    reinterpret_cast<double *>(b.field(b.theta, m)), b.nx[m],
+                                    b.ny[m], kind, x, y, b.disks, nd, r2, b.stream);
+    else
+        e = kg::launch_init<float>(reinterpret_cast<float *>(b.field(b.phi[c], m)),
+                                   reinterpret_cast<float *>(b.field(b.T[c], m)),
+                                   reinterpret_cast<float *>(b.field(b.theta, m)), b.nx[m],
+                                   b.ny[m], kind, x, y, b.disks, nd, r2, b.stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "batch init launch");
+    // the disk list is reused by the next member: wait before overwriting it
+    KG_HIP(hipStreamSynchronize(b.stream));
+    return KG_OK;
+}
+
+int copy_member_out(kg_batch_s *b, int m, void *base, void *dst, size_t bytes)
+{
+    if (!dst)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null destination");
+    if (bytes != b->cells(m) * b->esize())
+        return fail(KG_ERR_SIZE, "buffer size does not match the member's field size");
+    KG_HIP(hipSetDevice(b->device));
+    KG_HIP(hipMemcpyAsync(dst, b->field(base, m), bytes, hipMemcpyDeviceToHost, b->stream));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    return KG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kg_batch_create(int count, const int *nx, const int *ny, int precision,
+                    const kg_params *params, kg_batch *out)
+{
+    if (!out)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null output batch");
+    *out = nullptr;
+    if (count < 1 || count > (1 << 20) || !nx || !ny || !params)
+        return fail(KG_ERR_INVALID_ARGUMENT, "count must be in [1, 2^20] with nx, ny, params");
+    if (precision != KG_F32 && precision != KG_F64)
+        return fail(KG_ERR_INVALID_ARGUMENT, "precision must be KG_F32 or KG_F64");
+    for (int m = 0; m < count; ++m)
+        if (int rc = kg_validate(nx[m], ny[m], &params[m]))
+            return fail(rc, "member " + std::to_string(m) + ": " + kg_last_error());
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(KG_ERR_DEVICE, "no GPU visible to the HIP runtime");
+
+    kg_batch_s *b = new kg_batch_s();
+    b->count = count;
+    b->precision = precision;
+    b->nx.assign(nx, nx + count);
+    b->ny.assign(ny, ny + count);
+    b->params.assign(params, params + count);
+    const size_t align = 256 / b->esize();
+    size_t total = 0;
+    for (int m = 0; m < count; ++m) {
+        b->off.push_back(total);
+        total += (b->cells(m) + align - 1) / align * align;
+    }
+    b->total = total;
+    const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
+    std::vector<int4> tiles;
+    for (int m = 0; m < count; ++m)
+        for (int y0 = 0; y0 < ny[m]; y0 += TY)
+            for (int x0 = 0; x0 < nx[m]; x0 += TX)
+                tiles.push_back(make_int4(m, x0, y0, 0));
+    if (tiles.size() > (size_t)(1 << 30)) {
+        delete b;
+        return fail(KG_ERR_INVALID_ARGUMENT, "too many tiles");
+    }
+    b->ntiles = (int)tiles.size();
+
+    hipError_t e = hipGetDevice(&b->device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        free_batch(b);
+        return hip_fail(e, "stream/event creation");
+    }
+    const size_t fb = total * b->esize();
+    void **bufs[] = {&b->phi[0], &b->phi[1], &b->T[0], &b->T[1], &b->theta};
+    for (void **p : bufs)
+        if (hipMalloc(p, fb) != hipSuccess) {
+            *p = nullptr;
+            free_batch(b);
+            return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the batch fields failed");
+        }
+    const size_t args_bytes = (size_t)count * (precision == KG_F64 ? sizeof(kg::StepArgs<double>)
+                                                                    : sizeof(kg::StepArgs<float>));
+    if (hipMalloc(&b->members, args_bytes) != hipSuccess ||
+        hipMalloc(&b->tiles, tiles.size() * sizeof(int4)) != hipSuccess ||
+        hipMalloc(&b->step_base, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->partials, kg::reduce_partials_size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&b->red, sizeof(kg::ReduceOut)) != hipSuccess) {
+        free_batch(b);
+        return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the batch tables failed");
+    }
+    if (precision == KG_F64) {
+        auto v = member_args<double>(*b);
+        e = hipMemcpy(b->members, v.data(), args_bytes, hipMemcpyHostToDevice);
+    } else {
+        auto v = member_args<float>(*b);
+        e = hipMemcpy(b->members, v.data(), args_bytes, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemset(b->step_base, 0, sizeof(uint32_t));
+    if (e != hipSuccess) {
+        free_batch(b);
+        return hip_fail(e, "batch table upload");
+    }
+    for (int m = 0; m < count; ++m) {
+        int rc = init_member(*b, m, 0, nx[m] / 2, ny[m] / 2, {}, 0.0);
+        if (rc != KG_OK) {
+            free_batch(b);
+            return rc;
+        }
+    }
+    *out = b;
+    return KG_OK;
+}
+
+int kg_batch_destroy(kg_batch b)
+{
+    if (!b)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null batch");
+    free_batch(b);
+    return KG_OK;
+}
+
+int kg_batch_count(kg_batch b, int *count)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (!count)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null count");
+    *count = b->count;
+    return KG_OK;
+}
+
+int kg_batch_reset_disk(kg_batch b, int member, int cx, int cy, double r)
+{
+    if (int rc = check_batch(b, member, true))
+        return rc;
+    if (!(r >= 0.0) || !std::isfinite(r))
+        return fail(KG_ERR_INVALID_ARGUMENT, "radius must be finite and >= 0");
+    KG_HIP(hipSetDevice(b->device));
+    if (member < 0) {
+        b->steps = 0;  // every member restarts: counter 0, buffer 0
+        for (int m = 0; m < b->count; ++m) {
+            const int x = cx < 0 ? b->nx[m] / 2 : cx, y = cx < 0 ? b->ny[m] / 2 : cy;
+            if (int rc = init_member(*b, m, 1, 0, 0, {x, y}, r * r))
+                return rc;
+        }
+        return KG_OK;
+    }
+    return init_member(*b, member, 1, 0, 0,
+                       {cx < 0 ? b->nx[member] / 2 : cx, cx < 0 ? b->ny[member] / 2 : cy}, r * r);
+}
+
+int kg_batch_reset_seeds(kg_batch b, int member, int count, double r, int margin, uint64_t seed)
+{
+    if (int rc = check_batch(b, member, true))
+        return rc;
+    if (count < 0 || count > 1 << 16 || !(r >= 0.0) || !std::isfinite(r) || margin < 0)
+        return fail(KG_ERR_INVALID_ARGUMENT, "bad seed count, radius or margin");
+    KG_HIP(hipSetDevice(b->device));
+    const int lo = member < 0 ? 0 : member, hi = member < 0 ? b->count : member + 1;
+    for (int m = lo; m < hi; ++m)
+        if (2 * margin >= b->nx[m] || 2 * margin >= b->ny[m])
+            return fail(KG_ERR_INVALID_ARGUMENT, "margin leaves no room for centres");
+    if (member < 0)
+        b->steps = 0;
+    for (int m = lo; m < hi; ++m) {
+        std::vector<int> d(2 * (size_t)count);
+        kg::seed_centres(b->nx[m], b->ny[m], count, margin, seed, d.data());
+        if (int rc = init_member(*b, m, 1, 0, 0, d, r * r))
+            return rc;
+    }
+    return KG_OK;
+}
+
+int kg_batch_set_fields(kg_batch b, int member, const void *phi, const void *T,
+                        const void *theta, size_t bytes)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    if (!phi || !T)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null phi or T");
+    if (bytes != b->cells(member) * b->esize())
+        return fail(KG_ERR_SIZE, "buffer size does not match the member's field size");
+    KG_HIP(hipSetDevice(b->device));
+    const int c = b->cur();
+    KG_HIP(hipMemcpyAsync(b->field(b->phi[c], member), phi, bytes, hipMemcpyHostToDevice,
+                          b->stream));
+    KG_HIP(hipMemcpyAsync(b->field(b->T[c], member), T, bytes, hipMemcpyHostToDevice, b->stream));
+    if (theta)
+        KG_HIP(hipMemcpyAsync(b->field(b->theta, member), theta, bytes, hipMemcpyHostToDevice,
+                              b->stream));
+    else
+        KG_HIP(hipMemsetAsync(b->field(b->theta, member), 0, bytes, b->stream));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    return KG_OK;
+}
+
+int kg_batch_get_phase(kg_batch b, int member, void *dst, size_t bytes)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    return copy_member_out(b, member, b->phi[b->cur()], dst, bytes);
+}
+
+int kg_batch_get_temperature(kg_batch b, int member, void *dst, size_t bytes)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    return copy_member_out(b, member, b->T[b->cur()], dst, bytes);
+}
+
+int kg_batch_get_angle(kg_batch b, int member, void *dst, size_t bytes)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    return copy_member_out(b, member, b->theta, dst, bytes);
+}
+
+int kg_batch_reduce(kg_batch b, int member, kg_stats *out)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    if (!out)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null stats");
+    KG_HIP(hipSetDevice(b->device));
+    const int c = b->cur();
+    const size_t n = b->cells(member);
+    hipError_t e;
+    if (b->precision == KG_F64)
+        e = kg::launch_reduce<double>(reinterpret_cast<const double *>(b->field(b->phi[c], member)),
+                                      reinterpret_cast<const double *>(b->field(b->T[c], member)), n,
+                                      b->params[member].K, b->partials, b->red, b->stream);
+    else
+        e = kg::launch_reduce<float>(reinterpret_cast<const float *>(b->field(b->phi[c], member)),
+                                     reinterpret_cast<const float *>(b->field(b->T[c], member)), n,
+                                     (double)(float)b->params[member].K, b->partials, b->red,
+                                     b->stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "batch reduce launch");
+    kg::ReduceOut r{};
+    KG_HIP(hipMemcpyAsync(&r, b->red, sizeof(r), hipMemcpyDeviceToHost, b->stream));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    out->sum_phi = r.sum_phi;
+    out->sum_T = r.sum_T;
+    out->sum_T_minus_K_phi = r.sum_T_minus_K_phi;
+    out->min_phi = r.min_phi;
+    out->max_phi = r.max_phi;
+    out->solid_fraction = r.above_half / (double)n;
+    out->nonfinite = (int64_t)r.nonfinite;
+    out->cells = (int64_t)n;
+    return KG_OK;
+}
+
+int kg_batch_set_mode(kg_batch b, int mode, int graph_steps)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (mode != KG_BATCH_LAUNCH && mode != KG_BATCH_GRAPH)
+        return fail(KG_ERR_INVALID_ARGUMENT, "mode must be KG_BATCH_LAUNCH or KG_BATCH_GRAPH");
+    if (mode == KG_BATCH_GRAPH && (graph_steps < 2 || (graph_steps & 1) || graph_steps > 4096))
+        return fail(KG_ERR_INVALID_ARGUMENT, "graph_steps must be even and in [2, 4096]");
+    KG_HIP(hipSetDevice(b->device));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    if (mode != b->mode || graph_steps != b->graph_steps)
+        drop_graph(b);
+    b->mode = mode;
+    b->graph_steps = mode == KG_BATCH_GRAPH ? graph_steps : 0;
+    return KG_OK;
+}
+
+int kg_batch_get_mode(kg_batch b, int *mode, int *graph_steps)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (mode)
+        *mode = b->mode;
+    if (graph_steps)
+        *graph_steps = b->graph_steps;
+    return KG_OK;
+}
+
+int kg_batch_step(kg_batch b, int nsteps)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (nsteps < 0)
+        return fail(KG_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
+    KG_HIP(hipSetDevice(b->device));
+    return step_impl(*b, nsteps);
+}
+
+int kg_batch_step_timed(kg_batch b, int nsteps, double *ms)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (!ms || nsteps < 0)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null ms or negative nsteps");
+    KG_HIP(hipSetDevice(b->device));
+    // a graph replay may first upload the counter synchronously; do it
+    // outside the timed region
+    if (b->mode == KG_BATCH_GRAPH && nsteps >= b->graph_steps && b->base_host != b->steps) {
+        b->base_tmp = (uint32_t)b->steps;
+        KG_HIP(hipMemcpyAsync(b->step_base, &b->base_tmp, sizeof(uint32_t),
+                              hipMemcpyHostToDevice, b->stream));
+        KG_HIP(hipStreamSynchronize(b->stream));
+        b->base_host = b->steps;
+    }
+    KG_HIP(hipEventRecord(b->ev0, b->stream));
+    if (int rc = step_impl(*b, nsteps))
+        return rc;
+    KG_HIP(hipEventRecord(b->ev1, b->stream));
+    KG_HIP(hipEventSynchronize(b->ev1));
+    float t = 0.0f;
+    KG_HIP(hipEventElapsedTime(&t, b->ev0, b->ev1));
+    *ms = (double)t;
+    return KG_OK;
+}
+
+int kg_batch_synchronize(kg_batch b)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    KG_HIP(hipSetDevice(b->device));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    return KG_OK;
+}
+
+int kg_batch_get_step_count(kg_batch b, uint64_t *steps)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (!steps)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null steps");
+    *steps = b->steps;
+    return KG_OK;
+}
+
+int kg_batch_autotune(kg_batch b, int trial_steps, int *mode, int *graph_steps)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (trial_steps < 2)
+        return fail(KG_ERR_INVALID_ARGUMENT, "trial_steps must be >= 2");
+    KG_HIP(hipSetDevice(b->device));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    // save the state: the current buffers, theta and the counter
+    const size_t fb = b->total * b->esize();
+    const int c = b->cur();
+    void *save = nullptr;
+    if (hipMalloc(&save, 3 * fb) != hipSuccess)
+        return fail(KG_ERR_OUT_OF_MEMORY, "autotune needs room for a copy of the state");
+    char *s = static_cast<char *>(save);
+    hipError_t e = hipMemcpyAsync(s, b->phi[c], fb, hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s + fb, b->T[c], fb, hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s + 2 * fb, b->theta, fb, hipMemcpyDeviceToDevice, b->stream);
+    const uint64_t steps0 = b->steps;
+    const int mode0 = b->mode, g0 = b->graph_steps;
+    struct Cand { int mode, g; };
+    const Cand cands[] = {{KG_BATCH_LAUNCH, 0}, {KG_BATCH_GRAPH, 2}, {KG_BATCH_GRAPH, 8},
+                          {KG_BATCH_GRAPH, 32}};
+    double best = 1e300;
+    Cand pick = {mode0, g0};
+    for (const Cand &cd : cands) {
+        if (e != hipSuccess)
+            break;
+        if (cd.mode == KG_BATCH_GRAPH && cd.g > trial_steps)
+            continue;
+        int rc = kg_batch_set_mode(b, cd.mode, cd.g);
+        double ms = 0.0;
+        if (rc == KG_OK)
+            rc = kg_batch_step(b, cd.g > 0 ? cd.g : 2);  // warm up (capture, caches)
+        if (rc == KG_OK)
+            rc = kg_batch_step_timed(b, trial_steps, &ms);
+        // restore the state before the next candidate
+        b->steps = steps0;
+        const int cc = b->cur();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->phi[cc], s, fb, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->T[cc], s + fb, fb, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->theta, s + 2 * fb, fb, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(b->stream);
+        if (rc != KG_OK) {
+            (void)hipFree(save);
+            return rc;
+        }
+        if (ms < best) {
+            best = ms;
+            pick = cd;
+        }
+    }
+    (void)hipFree(save);
+    if (e != hipSuccess)
+        return hip_fail(e, "autotune state copy");
+    if (int rc = kg_batch_set_mode(b, pick.mode, pick.g))
+        return rc;
+    if (mode)
+        *mode = pick.mode;
+    if (graph_steps)
+        *graph_steps = pick.g;
+    return KG_OK;
+}
+
+}  // extern "C"

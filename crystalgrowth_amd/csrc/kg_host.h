@@ -1,0 +1,88 @@
+// kg_host.h -- host-side helpers shared by the C ABI sources: error state,
+// HIP status checks, and the StepArgs a kernel launch needs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/kobayashi_gpu.h"
+#include "kg_internal.h"
+
+namespace kg {
+struct AdjointWork;
+void free_adjoint(AdjointWork *w);
+}  // namespace kg
+
+// A solver handle (kg_handle): one grid's device fields and parameters.
+struct kg_solver_s {
+    int nx = 0, ny = 0, precision = KG_F32;
+    kg_params params{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *phi[2] = {nullptr, nullptr};
+    void *T[2] = {nullptr, nullptr};
+    void *theta = nullptr;
+    void *temps = nullptr;  // 6 fields, two-pass kernel only
+    double *partials = nullptr;
+    kg::ReduceOut *red = nullptr;
+    int *disks = nullptr;
+    int disks_cap = 0;
+    int cur = 0;
+    uint64_t steps = 0;
+    int kernel = KG_KERNEL_FUSED;
+    kg::AdjointWork *adj = nullptr;  // adjoint buffers, allocated on first use
+
+    size_t cells() const { return (size_t)nx * (size_t)ny; }
+    size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
+    size_t field_bytes() const { return cells() * esize(); }
+};
+
+namespace kg {
+
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define KG_HIP(call)                               \
+    do {                                           \
+        hipError_t e_ = (call);                    \
+        if (e_ != hipSuccess)                      \
+            return ::kg::hip_fail(e_, #call);      \
+    } while (0)
+
+// Scalar parameters combined in R, in the reference's association order
+// (see StepArgs); pointers and step counter filled in by the caller.
+template <typename R>
+StepArgs<R> make_args(const kg_params &p, int nx, int ny)
+{
+    StepArgs<R> a{};
+    a.nx = nx;
+    a.ny = ny;
+    a.boundary = p.boundary;
+    const R dx = (R)p.dx, dy = (R)p.dy, dt = (R)p.dt, tau = (R)p.tau;
+    a.inv_dx = (R)1 / dx;
+    a.inv_dy = (R)1 / dy;
+    a.inv_3dx2 = (R)1 / ((R)3 * dx * dx);
+    a.dt = dt;
+    a.dt_tau = dt / tau;
+    a.eps_bar = (R)p.eps_bar;
+    a.delta = (R)p.delta;
+    a.aniso = (R)p.aniso;
+    a.theta0 = (R)p.theta0;
+    a.neg_ebar_j_delta = -(R)p.eps_bar * (R)p.aniso * (R)p.delta;
+    a.alpha_pi = (R)p.alpha / (sizeof(R) == 4 ? (R)3.14159274f : (R)3.14159265358979323846);
+    a.gamma = (R)p.gamma;
+    a.teq = (R)p.teq;
+    a.K = (R)p.K;
+    a.noise_a = (R)p.noise_a;
+    a.instance = p.instance;
+    a.key0 = (uint32_t)p.seed;
+    a.key1 = (uint32_t)(p.seed >> 32);
+    return a;
+}
+
+// Centres of `count` Philox-drawn nuclei (kg_reset_seeds).
+void seed_centres(int nx, int ny, int count, int margin, uint64_t seed, int *cxy);
+
+}  // namespace kg

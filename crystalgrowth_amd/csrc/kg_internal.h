@@ -69,5 +69,15 @@ hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba,
 
 // Tiles of the fused kernel and its launch geometry.
 void fused_geometry(int nx, int ny, int *ntiles, int *per_xcd, int *nblocks);
+// Output tile size of the fused kernel (members of a batch are cut into it).
+int fused_tile_x();
+int fused_tile_y();
+
+// Batched step over a tile table (member, x0, y0); the step counter is
+// *step_base + step_add and its parity picks the ping-pong direction.
+template <typename R>
+hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
+                             const uint32_t *step_base, uint32_t step_add, hipStream_t s);
+hipError_t launch_advance(uint32_t *step_base, uint32_t n, hipStream_t s);
 
 }  // namespace kg

@@ -2,33 +2,48 @@
 //
 // One explicit-Euler step = the reference's pass 1 (_computeGradientLaplacian,
 // src/Kobayashi.cpp:125-175) and pass 2 (_evolution, :177-221).  The fused
-// kernel does both in one launch: a 64x16 output tile per 256-thread block,
-// phi staged in LDS with a 2-cell halo, T with a 1-cell halo, the pass-1
-// quantities that pass 2 reads from neighbours (eps^2, eps eps' gx,
-// eps eps' gy) computed once per cell of the tile plus a 1-cell halo and kept
-// in LDS.  phi and T ping-pong between two buffers; theta is one buffer,
-// written only where a branch of :154-167 fires and read only where none
-// does, so blocks never race on it.
+// kernel does both in one launch per step:
+//   * a 256-thread block owns a 64 x 16 output tile; phi (halo 2) and T
+//     (halo 1) are staged in LDS as rows x0-4 .. x0+67, loaded with aligned
+//     16-byte vector loads on interior tiles (mapped scalar loads on tiles
+//     touching the grid edge);
+//   * the pass-1 quantities that pass 2 reads from neighbours -- eps^2,
+//     eps eps' gx, eps eps' gy -- are computed once per cell of the tile plus
+//     a 1-cell halo (66 x 18 cells) and kept in LDS;
+//   * each thread then updates a row of 4 consecutive cells with 16-byte
+//     LDS reads, one Philox block for their 4 noise samples, and one 16-byte
+//     store per field.
+// phi and T ping-pong between two buffers; theta is one buffer, written only
+// where a branch of :154-167 fires and read only where none does, so blocks
+// never race on it.
 //
-// Memory-bound: per cell-update the algorithmic traffic is one read and one
-// write of phi and T (16 B in FP32).  Tiles whose phi is exactly 0 or exactly
-// 1 everywhere (plus halo) reduce to the T diffusion update, bit for bit the
-// same result as the full arithmetic (every cross term is a signed zero).
+// Memory-bound by design: per cell-update the algorithmic traffic is one
+// read and one write of phi and T (16 B in FP32).  Tiles whose phi is exactly
+// 0 or exactly 1 everywhere (halo included) reduce to the T diffusion update,
+// bit for bit the result of the full arithmetic (every cross term is a signed
+// zero); where phi (1 - phi) == 0 the reaction (m(T), noise) is skipped for
+// the same reason.
 #include <hip/hip_runtime.h>
 #include <float.h>
 #include <math.h>
 
 #include "kg_internal.h"
+#include "kg_math.h"
 #include "kg_philox.h"
 
 namespace kg {
 
 namespace {
 
-constexpr int TX = 64;   // output tile width (x, contiguous)
-constexpr int TY = 16;   // output tile height
-constexpr int NT = 256;  // threads per block (4 wave64)
-constexpr int NXCD = 8;  // MI355X: workgroups go round-robin over 8 XCDs
+constexpr int TX = 64;        // output tile width (x, contiguous)
+constexpr int TY = 16;        // output tile height
+constexpr int NT = 256;       // threads per block (4 wave64), one 4-cell row group each
+constexpr int LW = TX + 8;    // LDS row: columns x0-4 .. x0+67 (16-byte aligned)
+constexpr int PH = TY + 4;    // phi rows y0-2 .. y0+TY+1
+constexpr int QH = TY + 2;    // T and pass-1 rows y0-1 .. y0+TY
+constexpr int DW = TX + 2;    // pass-1 cells per row: x0-1 .. x0+64 (LDS columns 3 .. 68)
+constexpr int NXCD = 8;       // MI355X: workgroups go round-robin over 8 XCDs
+static_assert(NT == (TX / 4) * TY, "one thread per 4-cell row group");
 
 template <typename R> struct Consts;
 template <> struct Consts<float> {
@@ -39,14 +54,30 @@ template <> struct Consts<double> {
     static __device__ __forceinline__ double pi() { return 3.14159265358979323846; }
 };
 
-__device__ __forceinline__ float kg_atan(float x) { return atanf(x); }
-__device__ __forceinline__ double kg_atan(double x) { return atan(x); }
-__device__ __forceinline__ void kg_sincos(float x, float *s, float *c) { sincosf(x, s, c); }
-__device__ __forceinline__ void kg_sincos(double x, double *s, double *c) { sincos(x, s, c); }
-__device__ __forceinline__ float kg_div(float a, float b) { return __fdividef(a, b); }
-__device__ __forceinline__ double kg_div(double a, double b) { return a / b; }
 __device__ __forceinline__ bool is_pos_zero(float v) { return __float_as_uint(v) == 0u; }
 __device__ __forceinline__ bool is_pos_zero(double v) { return __double_as_longlong(v) == 0ll; }
+
+// 4 consecutive elements, 16-byte aligned (float) or 2 x 16 bytes (double)
+__device__ __forceinline__ void load4(const float *p, float v[4])
+{
+    const float4 q = *reinterpret_cast<const float4 *>(p);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ __forceinline__ void load4(const double *p, double v[4])
+{
+    const double2 a = reinterpret_cast<const double2 *>(p)[0];
+    const double2 b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void store4(float *p, const float v[4])
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void store4(double *p, const double v[4])
+{
+    reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
 
 template <int BC>
 __device__ __forceinline__ int map_index(int x, int n)
@@ -68,7 +99,7 @@ __device__ __forceinline__ bool theta_branch(R gx, R gy, R &th)
 {
     const R E = (R)1.1920928955078125e-7;  // FLT_EPSILON in both precisions
     const R PI = Consts<R>::pi();
-    const R t = kg_atan(kg_div(gy, gx));
+    const R t = kg_atan_ratio(gy, gx);     // atan(gy / gx)
     if (gx > E) {
         if (gy < -E) {
             th = (R)2.0 * PI + t;
@@ -97,38 +128,16 @@ __device__ __forceinline__ bool theta_branch(R gx, R gy, R &th)
     return false;
 }
 
+// (2 (E + W + N + S) + NE + SW + NW + SE - 12 C), the order of :142-151,
+// for centre index c of three rows (m = j-1, z = j, p = j+1)
 template <typename R>
-__device__ __forceinline__ R cell_noise(int i, int j, const StepArgs<R> &a)
+__device__ __forceinline__ R lap9(const R *m, const R *z, const R *p, int c)
 {
-    const u32x4 v = philox4x32_10(u32x4{(uint32_t)i >> 2, (uint32_t)j, a.step, a.instance},
-                                  a.key0, a.key1);
-    return noise_from_word<R>(select_word(v, (uint32_t)i & 3u));
+    return (R)2.0 * (z[c + 1] + z[c - 1] + p[c] + m[c]) + p[c + 1] + m[c - 1] + p[c - 1] +
+           m[c + 1] - (R)12.0 * z[c];
 }
 
-// (2 (E + W + N + S) + NE + SW + NW + SE - 12 C), the order of :142-151.
-template <typename R>
-__device__ __forceinline__ R lap9_sum(const R *s, int c, int w)
-{
-    return (R)2.0 * (s[c + 1] + s[c - 1] + s[c + w] + s[c - w]) + s[c + w + 1] + s[c - w - 1] +
-           s[c + w - 1] + s[c - w + 1] - (R)12.0 * s[c];
-}
-
-// Reaction + combination of :206-215 for one cell.
-template <typename R>
-__device__ __forceinline__ void update_cell(const StepArgs<R> &a, int i, int j, R phi, R T,
-                                            R lap, R lapT, R e2c, R t1, R t2, R t3, R &phin,
-                                            R &Tn)
-{
-    const R m = a.alpha_pi * kg_atan(a.gamma * (a.teq - T));
-    R q = phi - (R)0.5 + m;
-    if (a.noise_a != (R)0)
-        q = q + a.noise_a * cell_noise<R>(i, j, a);
-    const R sum = t1 + t2 + e2c * lap + t3 + phi * ((R)1.0 - phi) * q;
-    phin = phi + sum * a.dt_tau;
-    Tn = T + lapT * a.dt + a.K * (phin - phi);
-}
-
-__device__ __forceinline__ int xcd_tile(int ntiles, int per_xcd)
+__device__ __forceinline__ int xcd_tile(int per_xcd)
 {
     // Block b runs on XCD b % 8; give each XCD a contiguous run of tiles so
     // that halo rows shared by neighbouring tiles stay in one L2.
@@ -136,111 +145,274 @@ __device__ __forceinline__ int xcd_tile(int ntiles, int per_xcd)
     return (b % NXCD) * per_xcd + b / NXCD;
 }
 
-template <typename R, int BC>
-__global__ __launch_bounds__(NT) void k_step_fused(StepArgs<R> a)
-{
-    constexpr int PW = TX + 4, PH = TY + 4;  // phi tile, halo 2
-    constexpr int QW = TX + 2, QH = TY + 2;  // T and pass-1 tiles, halo 1
-    __shared__ R sP[PH * PW];
-    __shared__ R sT[QH * QW];
-    __shared__ R sE2[QH * QW];
-    __shared__ R sA[QH * QW];
-    __shared__ R sB[QH * QW];
+// LDS of one tile: phi (halo 2), T (halo 1) and the pass-1 values that pass 2
+// reads from neighbours, each row covering columns x0-4 .. x0+67.
+template <typename R>
+struct TileSmem {
+    R P[PH * LW];
+    R T[QH * LW];
+    R E2[QH * LW];
+    R A[QH * LW];
+    R B[QH * LW];
+};
 
-    const int tile = xcd_tile(a.ntiles, a.per_xcd);
-    if (tile >= a.ntiles)
-        return;
+// One explicit-Euler step of the 64 x 16 output tile at (x0, y0).
+template <typename R, int BC>
+__device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, TileSmem<R> &sm)
+{
+    R *__restrict__ sP = sm.P;
+    R *__restrict__ sT = sm.T;
+    R *__restrict__ sE2 = sm.E2;
+    R *__restrict__ sA = sm.A;
+    R *__restrict__ sB = sm.B;
     const int nx = a.nx, ny = a.ny;
-    const int nbx = (nx + TX - 1) / TX;
-    const int x0 = (tile % nbx) * TX, y0 = (tile / nbx) * TY;
     const int tid = threadIdx.x;
 
-    int all0 = 1, all1 = 1;
-    for (int k = tid; k < PH * PW; k += NT) {
-        const int ly = k / PW, lx = k - ly * PW;
-        const int gx = map_index<BC>(x0 + lx - 2, nx), gy = map_index<BC>(y0 + ly - 2, ny);
-        const R v = a.phi_in[(size_t)gy * nx + gx];
-        sP[k] = v;
-        all0 &= is_pos_zero(v);
-        all1 &= (v == (R)1.0);
-    }
-    int finite = 1;
-    for (int k = tid; k < QH * QW; k += NT) {
-        const int ly = k / QW, lx = k - ly * QW;
-        const int gx = map_index<BC>(x0 + lx - 1, nx), gy = map_index<BC>(y0 + ly - 1, ny);
-        const R v = a.T_in[(size_t)gy * nx + gx];
-        sT[k] = v;
-        finite &= isfinite(v) ? 1 : 0;
+    // ---- stage phi (rows y0-2..y0+TY+1) and T (rows y0-1..y0+TY) ---------
+    int all0 = 1, all1 = 1, finite = 1;
+    const bool interior = x0 >= 4 && x0 + TX + 4 <= nx && y0 >= 2 && y0 + TY + 2 <= ny &&
+                          (nx & 3) == 0;
+    if (interior) {
+        constexpr int V = LW / 4;
+        for (int k = tid; k < PH * V; k += NT) {
+            const int r = k / V, v = k - r * V;
+            R q[4];
+            load4(a.phi_in + (size_t)(y0 - 2 + r) * nx + (x0 - 4 + 4 * v), q);
+            store4(&sP[r * LW + 4 * v], q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                all0 &= is_pos_zero(q[e]);
+                all1 &= (q[e] == (R)1.0);
+            }
+        }
+        for (int k = tid; k < QH * V; k += NT) {
+            const int r = k / V, v = k - r * V;
+            R q[4];
+            load4(a.T_in + (size_t)(y0 - 1 + r) * nx + (x0 - 4 + 4 * v), q);
+            store4(&sT[r * LW + 4 * v], q);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                finite &= isfinite(q[e]) ? 1 : 0;
+        }
+    } else {
+        for (int k = tid; k < PH * LW; k += NT) {
+            const int r = k / LW, c = k - r * LW;
+            const R v = a.phi_in[(size_t)map_index<BC>(y0 - 2 + r, ny) * nx +
+                                 map_index<BC>(x0 - 4 + c, nx)];
+            sP[k] = v;
+            all0 &= is_pos_zero(v);
+            all1 &= (v == (R)1.0);
+        }
+        for (int k = tid; k < QH * LW; k += NT) {
+            const int r = k / LW, c = k - r * LW;
+            const R v = a.T_in[(size_t)map_index<BC>(y0 - 1 + r, ny) * nx +
+                               map_index<BC>(x0 - 4 + c, nx)];
+            sT[k] = v;
+            finite &= isfinite(v) ? 1 : 0;
+        }
     }
     const int flat0 = __syncthreads_and(all0 & finite);
     const int flat1 = __syncthreads_and(all1 & finite);
     const bool flat = flat0 || flat1;
 
+    // ---- pass 1 on the tile plus a 1-cell halo ----------------------------
     if (!flat) {
-        for (int k = tid; k < QH * QW; k += NT) {
-            const int ly = k / QW, lx = k - ly * QW;
-            const int ux = x0 + lx - 1, uy = y0 + ly - 1;  // unmapped global coords
-            int cx, cy, sx = lx, sy = ly;
+        for (int k = tid; k < QH * DW; k += NT) {
+            const int r = k / DW, c = k - r * DW;
+            const int ux = x0 - 1 + c, uy = y0 - 1 + r;  // unmapped global coords
+            int cx, cy, lc = c + 3, lr = r;              // LDS column / pass-1 row used
             if (BC == 1) {
                 // zero-flux: a ghost cell carries the pass-1 values of the
                 // boundary cell it clamps to (pass 2 clamps its indices too)
                 cx = ux < 0 ? 0 : (ux >= nx ? nx - 1 : ux);
                 cy = uy < 0 ? 0 : (uy >= ny ? ny - 1 : uy);
-                sx += cx - ux;
-                sy += cy - uy;
+                lc += cx - ux;
+                lr += cy - uy;
             } else {
                 cx = map_index<0>(ux, nx);
                 cy = map_index<0>(uy, ny);
             }
-            const int c = (sy + 1) * PW + (sx + 1);
-            const R gx = (sP[c + 1] - sP[c - 1]) * a.inv_dx;
-            const R gy = (sP[c + PW] - sP[c - PW]) * a.inv_dy;
+            const R *pz = &sP[(lr + 1) * LW];
+            const R gx = (pz[lc + 1] - pz[lc - 1]) * a.inv_dx;
+            const R gy = (sP[(lr + 2) * LW + lc] - sP[lr * LW + lc]) * a.inv_dy;
             const size_t gidx = (size_t)cy * nx + cx;
             R th;
             if (!theta_branch(gx, gy, th)) {
                 th = a.theta[gidx];
-            } else if (lx >= 1 && lx <= TX && ly >= 1 && ly <= TY && ux < nx && uy < ny) {
+            } else if (r >= 1 && r <= TY && c >= 1 && c <= TX && ux < nx && uy < ny) {
                 a.theta[gidx] = th;
             }
             R sn, cs;
             kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
             const R eps = a.eps_bar * ((R)1.0 + a.delta * cs);
             const R f = eps * (a.neg_ebar_j_delta * sn);
-            sE2[k] = eps * eps;
-            sA[k] = f * gx;
-            sB[k] = f * gy;
+            const int o = r * LW + c + 3;
+            sE2[o] = eps * eps;
+            sA[o] = f * gx;
+            sB[o] = f * gy;
         }
         __syncthreads();
     }
 
-    for (int k = tid; k < TX * TY; k += NT) {
-        const int ly = k / TX, lx = k - ly * TX;
-        const int i = x0 + lx, j = y0 + ly;
-        if (i >= nx || j >= ny)
-            continue;
-        const int cp = (ly + 2) * PW + (lx + 2);
-        const int cq = (ly + 1) * QW + (lx + 1);
-        const R phi = sP[cp], T = sT[cq];
-        const R lapT = lap9_sum(sT, cq, QW) * a.inv_3dx2;
-        R phin, Tn;
-        if (flat) {
-            phin = phi;
-            Tn = T + lapT * a.dt + a.K * (phin - phi);
-        } else {
-            const R lap = lap9_sum(sP, cp, PW) * a.inv_3dx2;
-            const R gx = (sP[cp + 1] - sP[cp - 1]) * a.inv_dx;
-            const R gy = (sP[cp + PW] - sP[cp - PW]) * a.inv_dy;
-            const R gEx = (sE2[cq + 1] - sE2[cq - 1]) * a.inv_dx;
-            const R gEy = (sE2[cq + QW] - sE2[cq - QW]) * a.inv_dy;
-            const R t1 = (sA[cq + QW] - sA[cq - QW]) * a.inv_dy;
-            const R t2 = -(sB[cq + 1] - sB[cq - 1]) * a.inv_dx;
-            const R t3 = gEx * gx + gEy * gy;
-            update_cell(a, i, j, phi, T, lap, lapT, sE2[cq], t1, t2, t3, phin, Tn);
+    // ---- pass 2: one row of 4 cells per thread ----------------------------
+    const int g = tid & (TX / 4 - 1), row = tid / (TX / 4);
+    const int i0 = x0 + 4 * g, j = y0 + row;
+    if (j >= ny || i0 >= nx)
+        return;
+    const int kc = 4 * g + 4;  // LDS column of the first cell
+    // 6-wide windows: index 0 = column i0-1, 1..4 = the cells, 5 = i0+4
+    R Pm[6], Pz[6], Pp[6], Tm[6], Tz[6], Tp[6];
+    auto win6 = [kc](const R *rowp, R w[6]) {
+        w[0] = rowp[kc - 1];
+        load4(rowp + kc, w + 1);
+        w[5] = rowp[kc + 4];
+    };
+    win6(&sP[(row + 1) * LW], Pm);
+    win6(&sP[(row + 2) * LW], Pz);
+    win6(&sP[(row + 3) * LW], Pp);
+    win6(&sT[row * LW], Tm);
+    win6(&sT[(row + 1) * LW], Tz);
+    win6(&sT[(row + 2) * LW], Tp);
+
+    R phin[4], Tn[4];
+    if (flat) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const R lapT = lap9(Tm, Tz, Tp, e + 1) * a.inv_3dx2;
+            phin[e] = Pz[e + 1];
+            Tn[e] = Tz[e + 1] + lapT * a.dt + a.K * (phin[e] - Pz[e + 1]);
         }
-        const size_t o = (size_t)j * nx + i;
-        a.phi_out[o] = phin;
-        a.T_out[o] = Tn;
+    } else {
+        R E2m[4], E2z[6], E2p[4], Am[4], Ap[4], Bz[6];
+        load4(&sE2[row * LW + kc], E2m);
+        win6(&sE2[(row + 1) * LW], E2z);
+        load4(&sE2[(row + 2) * LW + kc], E2p);
+        load4(&sA[row * LW + kc], Am);
+        load4(&sA[(row + 2) * LW + kc], Ap);
+        win6(&sB[(row + 1) * LW], Bz);
+
+        // the reaction matters only where phi (1 - phi) != 0 (or T is not
+        // finite, which the reference would turn into NaN)
+        bool need[4], any = false;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const R p = Pz[e + 1];
+            need[e] = (p * ((R)1.0 - p) != (R)0) || !isfinite(Tz[e + 1]);
+            any |= need[e];
+        }
+        R chi[4] = {(R)0, (R)0, (R)0, (R)0};
+        if (any && a.noise_a != (R)0) {
+            const u32x4 w = philox4x32_10(
+                u32x4{(uint32_t)i0 >> 2, (uint32_t)j, a.step, a.instance}, a.key0, a.key1);
+            chi[0] = noise_from_word<R>(w.x);
+            chi[1] = noise_from_word<R>(w.y);
+            chi[2] = noise_from_word<R>(w.z);
+            chi[3] = noise_from_word<R>(w.w);
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int c = e + 1;
+            const R phi = Pz[c], T = Tz[c];
+            const R lap = lap9(Pm, Pz, Pp, c) * a.inv_3dx2;
+            const R lapT = lap9(Tm, Tz, Tp, c) * a.inv_3dx2;
+            const R gx = (Pz[c + 1] - Pz[c - 1]) * a.inv_dx;
+            const R gy = (Pp[c] - Pm[c]) * a.inv_dy;
+            const R gEx = (E2z[c + 1] - E2z[c - 1]) * a.inv_dx;
+            const R gEy = (E2p[e] - E2m[e]) * a.inv_dy;
+            const R t1 = (Ap[e] - Am[e]) * a.inv_dy;
+            const R t2 = -(Bz[c + 1] - Bz[c - 1]) * a.inv_dx;
+            const R t3 = gEx * gx + gEy * gy;
+            R react = (R)0;
+            if (need[e]) {
+                const R m = a.alpha_pi * kg_atan(a.gamma * (a.teq - T));
+                R q = phi - (R)0.5 + m;
+                if (a.noise_a != (R)0)
+                    q = q + a.noise_a * chi[e];
+                react = phi * ((R)1.0 - phi) * q;
+            }
+            const R sum = t1 + t2 + E2z[c] * lap + t3 + react;
+            phin[e] = phi + sum * a.dt_tau;
+            Tn[e] = T + lapT * a.dt + a.K * (phin[e] - phi);
+        }
     }
+
+    const size_t o = (size_t)j * nx + i0;
+    if (i0 + 3 < nx && (nx & 3) == 0) {
+        store4(a.phi_out + o, phin);
+        store4(a.T_out + o, Tn);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (i0 + e < nx) {
+                a.phi_out[o + e] = phin[e];
+                a.T_out[o + e] = Tn[e];
+            }
+        }
+    }
+}
+
+template <typename R, int BC>
+__global__ __launch_bounds__(NT) void k_step_fused(StepArgs<R> a)
+{
+    __shared__ __align__(16) TileSmem<R> sm;
+    const int tile = xcd_tile(a.per_xcd);
+    if (tile >= a.ntiles)
+        return;
+    const int nbx = (a.nx + TX - 1) / TX;
+    step_tile<R, BC>(a, (tile % nbx) * TX, (tile / nbx) * TY, sm);
+}
+
+// Batched step: every tile of every ensemble member in one launch.  tiles[t]
+// = (member, x0, y0); members[m] holds the member's fields (buffer 0 as
+// phi_in/T_in, buffer 1 as phi_out/T_out) and parameters.  The step counter
+// is *step_base + step_add (the noise counter); its parity is the ping-pong
+// direction (a batch always restarts at step 0 in buffer 0), so a captured
+// graph replays correctly after its k_advance node moves *step_base on.
+template <typename R>
+__global__ __launch_bounds__(NT) void k_step_batch(const StepArgs<R> *__restrict__ members,
+                                                   const int4 *__restrict__ tiles, int ntiles,
+                                                   int per_xcd,
+                                                   const uint32_t *__restrict__ step_base,
+                                                   uint32_t step_add)
+{
+    __shared__ __align__(16) TileSmem<R> sm;
+    const int t = xcd_tile(per_xcd);
+    if (t >= ntiles)
+        return;
+    const int4 d = tiles[t];
+    StepArgs<R> a = members[d.x];
+    a.step = *step_base + step_add;
+    if (a.step & 1u) {
+        const R *pi = a.phi_in, *ti = a.T_in;
+        a.phi_in = a.phi_out;
+        a.T_in = a.T_out;
+        a.phi_out = const_cast<R *>(pi);
+        a.T_out = const_cast<R *>(ti);
+    }
+    if (a.boundary == 1)
+        step_tile<R, 1>(a, d.y, d.z, sm);
+    else
+        step_tile<R, 0>(a, d.y, d.z, sm);
+}
+
+__global__ void k_advance(uint32_t *step_base, uint32_t n) { *step_base += n; }
+
+// Reaction + combination of :206-215 for one cell (two-pass kernel).
+template <typename R>
+__device__ __forceinline__ void update_cell(const StepArgs<R> &a, int i, int j, R phi, R T,
+                                            R lap, R lapT, R e2c, R t1, R t2, R t3, R &phin,
+                                            R &Tn)
+{
+    const R m = a.alpha_pi * kg_atan(a.gamma * (a.teq - T));
+    R q = phi - (R)0.5 + m;
+    if (a.noise_a != (R)0) {
+        const u32x4 w = philox4x32_10(u32x4{(uint32_t)i >> 2, (uint32_t)j, a.step, a.instance},
+                                      a.key0, a.key1);
+        q = q + a.noise_a * noise_from_word<R>(select_word(w, (uint32_t)i & 3u));
+    }
+    const R sum = t1 + t2 + e2c * lap + t3 + phi * ((R)1.0 - phi) * q;
+    phin = phi + sum * a.dt_tau;
+    Tn = T + lapT * a.dt + a.K * (phin - phi);
 }
 
 // ---- two-pass variant: the reference's structure on the GPU -------------
@@ -488,6 +660,25 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s)
     return hipGetLastError();
 }
 
+int fused_tile_x() { return TX; }
+int fused_tile_y() { return TY; }
+
+template <typename R>
+hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
+                             const uint32_t *step_base, uint32_t step_add, hipStream_t s)
+{
+    const int per = (ntiles + NXCD - 1) / NXCD;
+    hipLaunchKernelGGL((k_step_batch<R>), dim3(per * NXCD), dim3(NT), 0, s, members, tiles,
+                       ntiles, per, step_base, step_add);
+    return hipGetLastError();
+}
+
+hipError_t launch_advance(uint32_t *step_base, uint32_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, step_base, n);
+    return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_step_two_pass(const StepArgs<R> &a, const PassTemps<R> &t, hipStream_t s)
 {
@@ -535,6 +726,8 @@ hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba,
 
 #define KG_INSTANTIATE(R)                                                                        \
     template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t);                  \
+    template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
+                                             const uint32_t *, uint32_t, hipStream_t);           \
     template hipError_t launch_step_two_pass<R>(const StepArgs<R> &, const PassTemps<R> &,       \
                                                 hipStream_t);                                    \
     template hipError_t launch_init<R>(R *, R *, R *, int, int, int, int, int, const int *, int, \

@@ -14,7 +14,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import KG_F32, KG_F64, KG_KERNEL_FUSED, KG_KERNEL_TWO_PASS, Params, Stats, check, lib
+from ._lib import (KG_BATCH_GRAPH, KG_BATCH_LAUNCH, KG_F32, KG_F64, KG_KERNEL_FUSED,
+                   KG_KERNEL_TWO_PASS, Grads, Params, Stats, check, lib)
 
 _PREC = {"f32": KG_F32, "float32": KG_F32, "f64": KG_F64, "float64": KG_F64}
 _KERNELS = {"fused": KG_KERNEL_FUSED, "two_pass": KG_KERNEL_TWO_PASS}
@@ -183,6 +184,24 @@ class Solver:
         check(lib().kg_colormap(self._h, out.ctypes.data_as(ctypes.c_void_p), out.nbytes, order))
         return out
 
+    # -- adjoint ----------------------------------------------------------
+    def adjoint(self, nsteps: int, target, checkpoint_every: int = 50, want_T0: bool = False,
+                want_phi0: bool = False):
+        """Loss sum (phi_N - target)^2 and its gradients from the current state
+        (kg_adjoint).  Returns (grads dict, dL/dT0 or None, dL/dphi0 or None);
+        the solver is left at the final forward state."""
+        tgt = np.ascontiguousarray(target, self.dtype)
+        if tgt.shape != (self.ny, self.nx):
+            raise ValueError("target shape must be (ny, nx)")
+        g = Grads()
+        dT = np.empty((self.ny, self.nx), self.dtype) if want_T0 else None
+        dP = np.empty((self.ny, self.nx), self.dtype) if want_phi0 else None
+        check(lib().kg_adjoint(self._h, int(nsteps), int(checkpoint_every),
+                               tgt.ctypes.data_as(ctypes.c_void_p), tgt.nbytes, ctypes.byref(g),
+                               None if dT is None else dT.ctypes.data_as(ctypes.c_void_p),
+                               None if dP is None else dP.ctypes.data_as(ctypes.c_void_p)))
+        return g.as_dict(), dT, dP
+
 
 class Kobayashi(Solver):
     """The reference class's simulation surface (src/Kobayashi.h:32-67) without
@@ -212,3 +231,114 @@ class Kobayashi(Solver):
     def iUpdateConstantBuffer(self) -> np.ndarray:
         """Colours of every constant buffer entry, in the reference's order."""
         return self.colormap(order=1)
+
+
+class Batch:
+    """An ensemble of independent grids of any sizes and parameters stepped
+    together (kg_batch_* of include/kobayashi_gpu.h).  Member m evolves bit for
+    bit like a Solver with the same size, precision, parameters and state."""
+
+    def __init__(self, sizes, params, precision: str = "f32"):
+        sizes = [(int(nx), int(ny)) for nx, ny in sizes]
+        if isinstance(params, Params):
+            params = [params] * len(sizes)
+        if len(params) != len(sizes):
+            raise ValueError("one kg_params per member")
+        n = len(sizes)
+        self.sizes = sizes
+        self.precision = _PREC[precision]
+        self.dtype = np.float64 if self.precision == KG_F64 else np.float32
+        nx = (ctypes.c_int * n)(*[s[0] for s in sizes])
+        ny = (ctypes.c_int * n)(*[s[1] for s in sizes])
+        arr = (Params * n)(*params)
+        self._h = ctypes.c_void_p()
+        check(lib().kg_batch_create(n, nx, ny, self.precision, arr, ctypes.byref(self._h)))
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def close(self) -> None:
+        if self._h:
+            check(lib().kg_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def reset_disk(self, member: int = -1, cx: int = -1, cy: int = -1, r: float = 5.0) -> None:
+        check(lib().kg_batch_reset_disk(self._h, member, cx, cy, float(r)))
+
+    def reset_seeds(self, count: int, r: float, margin: int, seed: int, member: int = -1) -> None:
+        check(lib().kg_batch_reset_seeds(self._h, member, count, float(r), margin, int(seed)))
+
+    def set_fields(self, member: int, phi, T, theta=None) -> None:
+        nx, ny = self.sizes[member]
+        a = [np.ascontiguousarray(x, self.dtype).reshape(ny, nx) for x in (phi, T)]
+        th = None if theta is None else np.ascontiguousarray(theta, self.dtype).reshape(ny, nx)
+        check(lib().kg_batch_set_fields(self._h, member, a[0].ctypes.data_as(ctypes.c_void_p),
+                                        a[1].ctypes.data_as(ctypes.c_void_p),
+                                        None if th is None else th.ctypes.data_as(ctypes.c_void_p),
+                                        a[0].nbytes))
+
+    def _get(self, fn, member):
+        nx, ny = self.sizes[member]
+        a = np.empty((ny, nx), self.dtype)
+        check(fn(self._h, member, a.ctypes.data_as(ctypes.c_void_p), a.nbytes))
+        return a
+
+    def phase(self, member: int) -> np.ndarray:
+        return self._get(lib().kg_batch_get_phase, member)
+
+    def temperature(self, member: int) -> np.ndarray:
+        return self._get(lib().kg_batch_get_temperature, member)
+
+    def angle(self, member: int) -> np.ndarray:
+        return self._get(lib().kg_batch_get_angle, member)
+
+    def fields(self, member: int):
+        return self.phase(member), self.temperature(member), self.angle(member)
+
+    def stats(self, member: int) -> dict:
+        s = Stats()
+        check(lib().kg_batch_reduce(self._h, member, ctypes.byref(s)))
+        return s.as_dict()
+
+    def set_mode(self, mode: str = "launch", graph_steps: int = 0) -> None:
+        m = {"launch": KG_BATCH_LAUNCH, "graph": KG_BATCH_GRAPH}[mode]
+        check(lib().kg_batch_set_mode(self._h, m, int(graph_steps)))
+
+    def mode(self):
+        m, g = ctypes.c_int(), ctypes.c_int()
+        check(lib().kg_batch_get_mode(self._h, ctypes.byref(m), ctypes.byref(g)))
+        return ("launch" if m.value == KG_BATCH_LAUNCH else "graph"), g.value
+
+    def autotune(self, trial_steps: int = 16):
+        m, g = ctypes.c_int(), ctypes.c_int()
+        check(lib().kg_batch_autotune(self._h, int(trial_steps), ctypes.byref(m), ctypes.byref(g)))
+        return ("launch" if m.value == KG_BATCH_LAUNCH else "graph"), g.value
+
+    def step(self, n: int = 1) -> None:
+        check(lib().kg_batch_step(self._h, int(n)))
+
+    def step_timed(self, n: int) -> float:
+        ms = ctypes.c_double()
+        check(lib().kg_batch_step_timed(self._h, int(n), ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self) -> None:
+        check(lib().kg_batch_synchronize(self._h))
+
+    @property
+    def step_count(self) -> int:
+        v = ctypes.c_uint64()
+        check(lib().kg_batch_get_step_count(self._h, ctypes.byref(v)))
+        return v.value

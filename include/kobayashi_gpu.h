@@ -154,6 +154,63 @@ int kg_reduce(kg_handle h, kg_stats *out);
  * _phi[_INDEX(i / n, i % n)] (square grids only). */
 int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order);
 
+/* ---- adjoint (reverse mode) ------------------------------------------------------------
+ * No counterpart in the reference (it has no gradients); needed to fit model
+ * parameters to a target morphology.  From the handle's current state and
+ * step counter, runs nsteps forward (storing (phi, T, theta) every
+ * checkpoint_every steps), evaluates L = sum (phi_N - target)^2 in FP64, and
+ * runs the hand-written reverse of each step (segments recomputed from their
+ * checkpoint) to accumulate dL/d(parameter) in FP64.  d_T0 and d_phi0 (host,
+ * nx*ny elements of the handle's precision, optional) receive dL/dT and dL/dphi
+ * of the initial state.  Afterwards the handle holds the final forward state.
+ * theta branches (src/Kobayashi.cpp:154-167) are piecewise: where a branch
+ * fires theta = const + atan(gy/gx) is differentiated, where none fires the
+ * kept theta passes its adjoint through. */
+typedef struct kg_grads {
+    double loss;
+    double d_tau, d_eps_bar, d_K, d_delta, d_theta0, d_alpha, d_gamma, d_teq, d_noise_a;
+} kg_grads;
+
+int kg_adjoint(kg_handle h, int nsteps, int checkpoint_every, const void *target, size_t bytes,
+               kg_grads *out, void *d_T0, void *d_phi0);
+
+/* ---- batched ensemble ------------------------------------------------------------------
+ * `count` independent members of any sizes and parameters (each its own
+ * kg_params, including seed and instance), stepped together: one launch per
+ * step covers every tile of every member, or a hipGraph replays a captured
+ * run of launches.  Member m of a batch evolves bit for bit like a plain
+ * handle with the same size, precision, parameters and initial state.
+ * No counterpart in the reference (one grid per process there); the
+ * per-member semantics are those of kg_step. */
+#define KG_BATCH_LAUNCH 0         /* one kernel launch per step */
+#define KG_BATCH_GRAPH 1          /* a captured hipGraph of graph_steps launches per replay */
+
+typedef struct kg_batch_s *kg_batch;
+
+int kg_batch_create(int count, const int *nx, const int *ny, int precision,
+                    const kg_params *params, kg_batch *out);
+int kg_batch_destroy(kg_batch b);
+int kg_batch_count(kg_batch b, int *count);
+/* member < 0 applies to every member (disk centred at (nx/2, ny/2) when cx < 0) */
+int kg_batch_reset_disk(kg_batch b, int member, int cx, int cy, double r);
+int kg_batch_reset_seeds(kg_batch b, int member, int count, double r, int margin, uint64_t seed);
+int kg_batch_set_fields(kg_batch b, int member, const void *phi, const void *T,
+                        const void *theta, size_t bytes);
+int kg_batch_get_phase(kg_batch b, int member, void *dst, size_t bytes);
+int kg_batch_get_temperature(kg_batch b, int member, void *dst, size_t bytes);
+int kg_batch_get_angle(kg_batch b, int member, void *dst, size_t bytes);
+int kg_batch_reduce(kg_batch b, int member, kg_stats *out);
+/* mode KG_BATCH_LAUNCH or KG_BATCH_GRAPH (graph_steps even, >= 2) */
+int kg_batch_set_mode(kg_batch b, int mode, int graph_steps);
+int kg_batch_get_mode(kg_batch b, int *mode, int *graph_steps);
+/* Times trial_steps steps of each launch configuration on a copy of the
+ * current state, keeps the fastest (the state is restored afterwards). */
+int kg_batch_autotune(kg_batch b, int trial_steps, int *mode, int *graph_steps);
+int kg_batch_step(kg_batch b, int nsteps);
+int kg_batch_step_timed(kg_batch b, int nsteps, double *ms);
+int kg_batch_synchronize(kg_batch b);
+int kg_batch_get_step_count(kg_batch b, uint64_t *steps);
+
 #ifdef __cplusplus
 }
 #endif

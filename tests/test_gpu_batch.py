@@ -1,0 +1,144 @@
+"""Batched ensemble (kg_batch_*) on the GPU: every member evolves bit for bit
+like a plain handle with the same size, precision, parameters and state, in
+per-step-launch and in hipGraph-replay mode."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def kb():
+    import crystalgrowth_amd as kb
+    if kb.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return kb
+
+
+def members(kb):
+    sizes = [(128, 128), (37, 23), (200, 96), (64, 64), (130, 70), (5, 4)]
+    params = [
+        kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1, seed=1, instance=0),
+        kb.default_params(delta=0.02, aniso=4.0, noise_a=0.02, seed=2, instance=1),
+        kb.default_params(delta=0.05, boundary=1, seed=3, instance=2),
+        kb.default_params(aniso=5.0, theta0=0.7, noise_a=0.01, seed=4, instance=3),
+        kb.default_params(delta=0.06, K=0.9, eps_bar=0.012, boundary=0, noise_a=0.01, seed=5, instance=4),
+        kb.default_params(boundary=1),
+    ]
+    return sizes, params
+
+
+def initial(sizes, prec):
+    dt = np.float32 if prec == "f32" else np.float64
+    out = []
+    rng = np.random.default_rng(7)
+    for k, (nx, ny) in enumerate(sizes):
+        if k in (2, 5):
+            phi = rng.random((ny, nx))
+            T = rng.random((ny, nx)) * 0.5
+            th = rng.uniform(0, 6, (ny, nx))
+        else:
+            yy, xx = np.mgrid[0:ny, 0:nx]
+            phi = (((xx - nx // 2) ** 2 + (yy - ny // 2) ** 2) <= 25).astype(float)
+            T = np.zeros((ny, nx))
+            th = np.zeros((ny, nx))
+        out.append(tuple(a.astype(dt) for a in (phi, T, th)))
+    return out
+
+
+def plain(kb, nx, ny, prec, p, state, nsteps):
+    with kb.Solver(nx, ny, prec, p) as s:
+        s.set_fields(*state)
+        s.step(nsteps)
+        return s.fields()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("mode,g", [("launch", 0), ("graph", 4)])
+def test_members_match_plain_solver_bitwise(kb, prec, mode, g):
+    sizes, params = members(kb)
+    init = initial(sizes, prec)
+    with kb.Batch(sizes, params, prec) as b:
+        if mode == "graph":
+            b.set_mode("graph", g)
+        for m, st in enumerate(init):
+            b.set_fields(m, *st)
+        b.step(11)  # 2 replays of 4 + 3 single launches in graph mode
+        assert b.step_count == 11
+        for m, (nx, ny) in enumerate(sizes):
+            ref = plain(kb, nx, ny, prec, params[m], init[m], 11)
+            got = b.fields(m)
+            for x, y in zip(got, ref):
+                assert np.array_equal(x, y), (m, nx, ny)
+
+
+def test_graph_replays_each_match_eager(kb):
+    """Replays with eager readbacks and synchronizations in between: every
+    replay advances the noise counter and matches the plain solver."""
+    sizes, params = members(kb)
+    init = initial(sizes, "f32")
+    solvers = []
+    for m, (nx, ny) in enumerate(sizes):
+        s = kb.Solver(nx, ny, "f32", params[m])
+        s.set_fields(*init[m])
+        solvers.append(s)
+    with kb.Batch(sizes, params, "f32") as b:
+        b.set_mode("graph", 6)
+        for m, st in enumerate(init):
+            b.set_fields(m, *st)
+        for rep in range(4):
+            b.step(6)
+            b.synchronize()
+            for m, s in enumerate(solvers):
+                s.step(6)
+                assert np.array_equal(b.phase(m), s.phase()), (rep, m)
+                assert np.array_equal(b.temperature(m), s.temperature()), (rep, m)
+            st = b.stats(0)           # eager work between replays
+            assert st["nonfinite"] == 0
+        # an odd remainder, then graph replays again from odd parity
+        b.step(1)
+        b.step(6)
+        for m, s in enumerate(solvers):
+            s.step(7)
+            assert np.array_equal(b.phase(m), s.phase()), m
+    for s in solvers:
+        s.close()
+
+
+def test_autotune_restores_state(kb):
+    sizes, params = members(kb)
+    init = initial(sizes, "f32")
+    with kb.Batch(sizes, params, "f32") as b:
+        for m, st in enumerate(init):
+            b.set_fields(m, *st)
+        b.step(3)
+        before = [b.fields(m) for m in range(len(sizes))]
+        mode, g = b.autotune(8)
+        assert mode in ("launch", "graph") and (mode == "launch" or g >= 2)
+        assert b.step_count == 3
+        for m in range(len(sizes)):
+            for x, y in zip(b.fields(m), before[m]):
+                assert np.array_equal(x, y)
+        b.step(5)
+        for m, (nx, ny) in enumerate(sizes):
+            ref = plain(kb, nx, ny, "f32", params[m], init[m], 8)
+            assert np.array_equal(b.phase(m), ref[0]), m
+
+
+def test_resets_and_reduce(kb):
+    sizes, params = members(kb)
+    with kb.Batch(sizes, params, "f32") as b:
+        b.reset_disk(-1, -1, -1, 5.0)
+        assert b.step_count == 0
+        b.step(20)
+        for m, (nx, ny) in enumerate(sizes):
+            with kb.Solver(nx, ny, "f32", params[m]) as s:
+                s.reset_disk(nx // 2, ny // 2, 5.0)
+                s.step(20)
+                assert np.array_equal(b.phase(m), s.phase())
+                assert b.stats(m) == s.stats()
+        b.reset_seeds(3, 2.0, 1, 99, member=0)  # one member: the counter continues
+        assert b.step_count == 20
+        with kb.Solver(*sizes[0], "f32", params[0]) as s:
+            s.reset_seeds(3, 2.0, 1, 99)
+            assert np.array_equal(b.phase(0), s.phase())
