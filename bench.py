@@ -15,7 +15,14 @@ region; `roofline` prices the fused step kernel's average launch (HIP events
 on the solver's own stream) at 16 algorithmic bytes per cell-update against
 the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the reference's
 arithmetic restated in C, one thread) on a bounded sample of the same
-workload.
+workload.  Beside the headline it reports:
+  batched   -- the 512-member ensemble (sides 128..1024, per-member
+               parameters), a shortened run: one batched launch per step
+               (autotuned launch mode) vs looping the plain solver per member
+  adjoint   -- loss + gradients of 1000 steps on 512^2 (checkpoint every 50)
+               vs the plain forward of the same steps
+--calibrate-copy runs only a device copy of known size (for scaling the
+rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes).
 """
 from __future__ import annotations
 
@@ -35,7 +42,7 @@ METRIC = "Gcell-updates/s & %HBM roofline; batched/adjoint vs plain at stated FP
 HBM_PEAK_GBS = 8000.0
 ALG_BYTES_FP32 = 16  # phi and T read once and written once per cell-update
 WORKLOAD = dict(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1,
-                seed=0)
+                seed=0)            # the 4096^2 configuration (crystalgrowth_amd/configs.py c1)
 SEEDS = dict(count=9, r=5.0, margin=8, seed=0)
 
 
@@ -49,6 +56,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="steps of the CPU oracle sample (0 skips the CPU baseline)")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=200,
+                    help="steps of the batched-vs-looped comparison (0 skips it)")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true")
     return ap.parse_args()
 
 
@@ -115,8 +126,88 @@ def fp64_error(kb, size: int, nsteps: int = 100):
     return out
 
 
+def calibrate_copy():
+    """Copy 2 x 64 MiB (the FP32 phi and T of a 4096^2 grid) 20 times."""
+    import torch
+    n = 2 * 4096 * 4096
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(20):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read_per_copy": 4 * n,
+                      "bytes_written_per_copy": 4 * n, "copies": 20}), flush=True)
+
+
+def batched_vs_looped(kb, nsteps: int):
+    from crystalgrowth_amd import configs
+    sizes, params = configs.c2_members()
+    cells = sum(a * b for a, b in sizes)
+    with kb.Batch(sizes, params, "f32") as b:
+        b.reset_disk(-1, -1, -1, 5.0)
+        mode, g = b.autotune(32)
+        b.reset_disk(-1, -1, -1, 5.0)
+        b.step(8)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(nsteps)
+        b.synchronize()
+        tb = time.perf_counter() - t0
+    solvers = [kb.Solver(nx, ny, "f32", p) for (nx, ny), p in zip(sizes, params)]
+    for (nx, ny), s in zip(sizes, solvers):
+        s.reset_disk(nx // 2, ny // 2, 5.0)
+        s.step(8)
+    for s in solvers:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step(nsteps)
+    for s in solvers:
+        s.synchronize()
+    tl = time.perf_counter() - t0
+    for s in solvers:
+        s.close()
+    return {"members": len(sizes), "cells": cells, "steps": nsteps,
+            "batched_Gcell_updates_per_s": round(cells * nsteps / tb / 1e9, 3),
+            "looped_Gcell_updates_per_s": round(cells * nsteps / tl / 1e9, 3),
+            "speedup_vs_looped": round(tl / tb, 3), "launch_mode": [mode, g],
+            "note": "per-member parity with the plain solver is bitwise (tests/test_gpu_batch.py)"}
+
+
+def adjoint_vs_forward(kb):
+    from crystalgrowth_amd import configs
+    c = configs.C3
+    n = c["n"]
+    with kb.Solver(n, n, "f32", configs.c3_params("target")) as s:
+        s.reset_disk(n // 2, n // 2, 5.0)
+        s.step(c["steps"])
+        target = s.phase()
+    with kb.Solver(n, n, "f32", configs.c3_params("guess")) as s:
+        s.reset_disk(n // 2, n // 2, 5.0)
+        s.adjoint(2 * c["every"], target, c["every"])  # allocate the work buffers
+        s.reset_disk(n // 2, n // 2, 5.0)
+        t0 = time.perf_counter()
+        g, _, _ = s.adjoint(c["steps"], target, c["every"], want_T0=True)
+        ta = time.perf_counter() - t0
+        s.reset_disk(n // 2, n // 2, 5.0)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(c["steps"])
+        s.synchronize()
+        tf = time.perf_counter() - t0
+    return {"grid": [n, n], "steps": c["steps"], "checkpoint_every": c["every"],
+            "adjoint_s": round(ta, 4), "forward_s": round(tf, 4),
+            "adjoint_over_forward": round(ta / tf, 2), "loss": g["loss"],
+            "grads": {k: g[k] for k in ("d_delta", "d_K", "d_eps_bar", "d_tau")},
+            "note": "gradients checked against central differences in tests/test_gpu_adjoint.py"}
+
+
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(json.dumps({"error": "this solver runs on one GPU: --gpus must be 1 (the grid "
@@ -179,6 +270,10 @@ def main():
     }
     if not args.no_fp64_check:
         rec["fp64_rel_err_100_steps"] = fp64_error(kb, n)
+    if args.batch_steps > 0:
+        rec["batched"] = batched_vs_looped(kb, args.batch_steps)
+    if not args.no_adjoint:
+        rec["adjoint"] = adjoint_vs_forward(kb)
     rec["cpu_baseline"] = cpu_baseline(n, args.cpu_steps) if args.cpu_steps > 0 else None
     print(json.dumps(rec), flush=True)
 

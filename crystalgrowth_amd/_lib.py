@@ -124,6 +124,9 @@ SIGNATURES = {
     "kg_batch_create": [I, PI, PI, I, P, ctypes.POINTER(H)],
     "kg_batch_destroy": [H],
     "kg_batch_count": [H, PI],
+    "kg_batch_set_params": [H, I, P],
+    "kg_batch_get_params": [H, I, P],
+    "kg_batch_adjoint": [H, I, I, ctypes.POINTER(VP), ctypes.POINTER(Grads), ctypes.POINTER(VP)],
     "kg_batch_reset_disk": [H, I, I, I, ctypes.c_double],
     "kg_batch_reset_seeds": [H, I, I, ctypes.c_double, I, U64],
     "kg_batch_set_fields": [H, I, VP, VP, VP, SZ],

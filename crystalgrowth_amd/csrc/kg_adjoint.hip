@@ -23,8 +23,10 @@
 // same fused kernel (bitwise identical) and walks them backwards.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -36,8 +38,8 @@ namespace kg {
 
 namespace {
 
-constexpr int ADJ_BLOCKS = 1024;
 constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_CHUNK = 1024;  // cells per block (4 per thread)
 enum { P_TAU, P_EPS_BAR, P_K, P_DELTA, P_THETA0, P_ALPHA, P_GAMMA, P_TEQ, P_NOISE, NPAR };
 
 template <typename R>
@@ -55,6 +57,36 @@ struct AdjIO {
     const R *lphi1, *lT1, *lth1;    // adjoint of the step's outputs
     R *lphi0, *lT0, *lth0;          // adjoint of the step's inputs
 };
+
+// One grid of the (possibly batched) adjoint: its step parameters and the
+// element offset of its fields in every buffer.  Blocks take chunks (member,
+// first cell, cell count, accumulator slot) from a table.
+template <typename R>
+struct AdjMember {
+    StepArgs<R> a;
+    size_t off;
+    R tau;
+};
+
+template <typename R>
+__device__ __forceinline__ AdjIO<R> shift(AdjIO<R> io, size_t o)
+{
+    io.phi += o; io.T += o; io.theta_old += o;
+    io.lphi1 += o; io.lT1 += o; io.lth1 += o;
+    io.lphi0 += o; io.lT0 += o; io.lth0 += o;
+    return io;
+}
+
+template <typename R>
+__device__ __forceinline__ AdjTemps<R> shift(AdjTemps<R> t, size_t o)
+{
+    R **f[] = {&t.gx, &t.gy, &t.lap, &t.lapT, &t.th, &t.eps, &t.epsd, &t.cs, &t.sn,
+               &t.ls, &t.gEx, &t.gEy, &t.lphi_loc, &t.lT_loc, &t.lgx, &t.lgy, &t.lth_old};
+    for (R **q : f)
+        *q += o;
+    t.code += o;
+    return t;
+}
 
 template <int BC>
 __device__ __forceinline__ int wrap(int x, int n)
@@ -112,7 +144,7 @@ __device__ __forceinline__ int theta_class(R gx, R gy, R &th)
 }
 
 template <typename R>
-__device__ __forceinline__ void block_add(double (&v)[NPAR], double *acc)
+__device__ __forceinline__ void block_add(double (&v)[NPAR], double *acc, int slot, int nslots)
 {
     __shared__ double sh[NPAR][ADJ_THREADS];
     for (int p = 0; p < NPAR; ++p)
@@ -125,18 +157,16 @@ __device__ __forceinline__ void block_add(double (&v)[NPAR], double *acc)
         __syncthreads();
     }
     if ((int)threadIdx.x < NPAR)
-        acc[(size_t)threadIdx.x * ADJ_BLOCKS + blockIdx.x] += sh[threadIdx.x][0];
+        acc[(size_t)threadIdx.x * nslots + slot] += sh[threadIdx.x][0];
 }
 
 // A1: pass-1 quantities of the forward step (same arithmetic as the kernels)
 template <typename R, int BC>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_pass1(StepArgs<R> a, AdjIO<R> io,
-                                                           AdjTemps<R> t)
+__device__ __forceinline__ void adj_pass1(const StepArgs<R> &a, const AdjIO<R> &io,
+                                          const AdjTemps<R> &t, size_t c0, size_t c1)
 {
     const int nx = a.nx, ny = a.ny;
-    const size_t n = (size_t)nx * ny;
-    for (size_t c = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x; c < n;
-         c += (size_t)ADJ_BLOCKS * ADJ_THREADS) {
+    for (size_t c = c0 + threadIdx.x; c < c1; c += ADJ_THREADS) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         const int ip = wrap<BC>(i + 1, nx), im = wrap<BC>(i - 1, nx);
         const int jp = wrap<BC>(j + 1, ny), jm = wrap<BC>(j - 1, ny);
@@ -169,14 +199,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_pass1(StepArgs<R> a, AdjIO<
 
 // A2: reverse of pass 2 at each cell (forward neighbour maps)
 template <typename R, int BC>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_pass2(StepArgs<R> a, AdjIO<R> io,
-                                                           AdjTemps<R> t, R tau, double *acc)
+__device__ __forceinline__ void adj_pass2(const StepArgs<R> &a, const AdjIO<R> &io,
+                                          const AdjTemps<R> &t, R tau, size_t c0, size_t c1,
+                                          double (&v)[NPAR])
 {
     const int nx = a.nx, ny = a.ny;
-    const size_t n = (size_t)nx * ny;
-    double v[NPAR] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (size_t c = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x; c < n;
-         c += (size_t)ADJ_BLOCKS * ADJ_THREADS) {
+    for (size_t c = c0 + threadIdx.x; c < c1; c += ADJ_THREADS) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         const size_t cE = (size_t)j * nx + wrap<BC>(i + 1, nx);
         const size_t cW = (size_t)j * nx + wrap<BC>(i - 1, nx);
@@ -213,22 +241,21 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_pass2(StepArgs<R> a, AdjIO<
         t.ls[c] = ls;
         t.gEx[c] = gEx;
         t.gEy[c] = gEy;
-        t.lphi_loc[c] = lp1 - a.K * lT1 + ls * (((R)1.0 - (R)2.0 * phi) * q + w1);
+        // phi enters phi' directly (coefficient 1) and through s; T' = T + lapT dt
+        // + K s dt/tau has no other phi dependence (its K lT' is inside ls)
+        t.lphi_loc[c] = lp1 + ls * (((R)1.0 - (R)2.0 * phi) * q + w1);
         t.lT_loc[c] = lT1 - ls * w1 * a.alpha_pi * a.gamma / d;
     }
-    block_add<R>(v, acc);
 }
 
 // A3: reverse of the pass-1 quantities at each cell (inverse neighbour maps)
 template <typename R, int BC>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_cell(StepArgs<R> a, AdjIO<R> io,
-                                                          AdjTemps<R> t, double *acc)
+__device__ __forceinline__ void adj_cell(const StepArgs<R> &a, const AdjIO<R> &io,
+                                         const AdjTemps<R> &t, size_t c0, size_t c1,
+                                         double (&v)[NPAR])
 {
     const int nx = a.nx, ny = a.ny;
-    const size_t n = (size_t)nx * ny;
-    double v[NPAR] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (size_t c = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x; c < n;
-         c += (size_t)ADJ_BLOCKS * ADJ_THREADS) {
+    for (size_t c = c0 + threadIdx.x; c < c1; c += ADJ_THREADS) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         R le2 = t.ls[c] * t.lap[c], lA = (R)0, lB = (R)0;
         int ks[2];
@@ -282,18 +309,15 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_cell(StepArgs<R> a, AdjIO<R
         v[P_EPS_BAR] += (double)(le2 * (R)2.0 * eps * eps / a.eps_bar) + (double)(G * (R)2.0 * F / a.eps_bar);
         v[P_THETA0] -= (double)(le2 * (R)2.0 * eps * epsd) + (double)(G * F_th);
     }
-    block_add<R>(v, acc);
 }
 
 // A4: gather of the input adjoints
 template <typename R, int BC>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_gather(StepArgs<R> a, AdjIO<R> io,
-                                                            AdjTemps<R> t)
+__device__ __forceinline__ void adj_gather(const StepArgs<R> &a, const AdjIO<R> &io,
+                                           const AdjTemps<R> &t, size_t c0, size_t c1)
 {
     const int nx = a.nx, ny = a.ny;
-    const size_t n = (size_t)nx * ny;
-    for (size_t c = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x; c < n;
-         c += (size_t)ADJ_BLOCKS * ADJ_THREADS) {
+    for (size_t c = c0 + threadIdx.x; c < c1; c += ADJ_THREADS) {
         const int i = (int)(c % nx), j = (int)(c / nx);
         R lp = t.lphi_loc[c], lT = t.lT_loc[c];
         int kx[2], ky[2];
@@ -328,19 +352,51 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_gather(StepArgs<R> a, AdjIO
     }
 }
 
-// loss = sum (phi - target)^2 in FP64; lphi = 2 (phi - target), lT = ltheta = 0
-template <typename R>
-__global__ __launch_bounds__(ADJ_THREADS) void k_adj_seed(const R *phi, const R *target, size_t n,
-                                                          R *lphi, R *lT, R *lth, double *partials)
+// The four reverse kernels: each block takes one chunk of one member.
+template <typename R, int KIND>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj(const AdjMember<R> *__restrict__ members,
+                                                     const int4 *__restrict__ chunks, AdjIO<R> io,
+                                                     AdjTemps<R> t, double *acc, int nslots,
+                                                     uint32_t step)
 {
+    const int4 ch = chunks[blockIdx.x];
+    AdjMember<R> m = members[ch.x];
+    m.a.step = step;  // the noise counter of the step being reversed
+    const AdjIO<R> mio = shift(io, m.off);
+    const AdjTemps<R> mt = shift(t, m.off);
+    const size_t c0 = (size_t)ch.y, c1 = c0 + (size_t)ch.z;
+    double v[NPAR] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const bool zf = m.a.boundary == 1;
+    if (KIND == 0) {
+        if (zf) adj_pass1<R, 1>(m.a, mio, mt, c0, c1); else adj_pass1<R, 0>(m.a, mio, mt, c0, c1);
+    } else if (KIND == 1) {
+        if (zf) adj_pass2<R, 1>(m.a, mio, mt, m.tau, c0, c1, v); else adj_pass2<R, 0>(m.a, mio, mt, m.tau, c0, c1, v);
+    } else if (KIND == 2) {
+        if (zf) adj_cell<R, 1>(m.a, mio, mt, c0, c1, v); else adj_cell<R, 0>(m.a, mio, mt, c0, c1, v);
+    } else {
+        if (zf) adj_gather<R, 1>(m.a, mio, mt, c0, c1); else adj_gather<R, 0>(m.a, mio, mt, c0, c1);
+    }
+    if (KIND == 1 || KIND == 2)
+        block_add<R>(v, acc, ch.w, nslots);
+}
+
+// loss partial of each chunk = sum (phi - target)^2 in FP64; lphi = 2 (phi -
+// target), lT = ltheta = 0
+template <typename R>
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_seed(const AdjMember<R> *__restrict__ members,
+                                                          const int4 *__restrict__ chunks,
+                                                          const R *phi, const R *target, R *lphi,
+                                                          R *lT, R *lth, double *partials)
+{
+    const int4 ch = chunks[blockIdx.x];
+    const size_t o = members[ch.x].off;
     double s = 0.0;
-    for (size_t c = (size_t)blockIdx.x * ADJ_THREADS + threadIdx.x; c < n;
-         c += (size_t)ADJ_BLOCKS * ADJ_THREADS) {
-        const R d = phi[c] - target[c];
+    for (size_t c = (size_t)ch.y + threadIdx.x; c < (size_t)ch.y + ch.z; c += ADJ_THREADS) {
+        const R d = phi[o + c] - target[o + c];
         s += (double)d * (double)d;
-        lphi[c] = (R)2.0 * d;
-        lT[c] = (R)0;
-        lth[c] = (R)0;
+        lphi[o + c] = (R)2.0 * d;
+        lT[o + c] = (R)0;
+        lth[o + c] = (R)0;
     }
     __shared__ double sh[ADJ_THREADS];
     sh[threadIdx.x] = s;
@@ -354,38 +410,43 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_seed(const R *phi, const R 
         partials[blockIdx.x] = sh[0];
 }
 
-template <typename R, int BC>
-hipError_t reverse_step(const StepArgs<R> &a, const AdjIO<R> &io, const AdjTemps<R> &t, R tau,
-                        double *acc, hipStream_t s)
+template <typename R>
+hipError_t reverse_step(const AdjMember<R> *members, const int4 *chunks, int nchunks,
+                        const AdjIO<R> &io, const AdjTemps<R> &t, double *acc, uint32_t step,
+                        hipStream_t s)
 {
-    const dim3 g(ADJ_BLOCKS), b(ADJ_THREADS);
-    hipLaunchKernelGGL((k_adj_pass1<R, BC>), g, b, 0, s, a, io, t);
-    hipLaunchKernelGGL((k_adj_pass2<R, BC>), g, b, 0, s, a, io, t, tau, acc);
-    hipLaunchKernelGGL((k_adj_cell<R, BC>), g, b, 0, s, a, io, t, acc);
-    hipLaunchKernelGGL((k_adj_gather<R, BC>), g, b, 0, s, a, io, t);
+    const dim3 g(nchunks), b(ADJ_THREADS);
+    hipLaunchKernelGGL((k_adj<R, 0>), g, b, 0, s, members, chunks, io, t, acc, nchunks, step);
+    hipLaunchKernelGGL((k_adj<R, 1>), g, b, 0, s, members, chunks, io, t, acc, nchunks, step);
+    hipLaunchKernelGGL((k_adj<R, 2>), g, b, 0, s, members, chunks, io, t, acc, nchunks, step);
+    hipLaunchKernelGGL((k_adj<R, 3>), g, b, 0, s, members, chunks, io, t, acc, nchunks, step);
     return hipGetLastError();
 }
 
 }  // namespace
 
 struct AdjointWork {
-    size_t n = 0;
+    size_t total = 0;        // elements per field buffer
     size_t esize = 0;
-    int ckpt_cap = 0, seg_cap = 0;
+    int ckpt_cap = 0, seg_cap = 0, nmembers = 0, nchunks = 0;
     char *ckpt = nullptr;    // ckpt_cap states of 3 fields
     char *seg = nullptr;     // seg_cap + 1 states of 3 fields
     char *lam = nullptr;     // 2 x (lphi, lT, ltheta)
     char *tmp = nullptr;     // NTEMP fields + target
     signed char *code = nullptr;
-    double *acc = nullptr;   // NPAR x ADJ_BLOCKS
+    double *acc = nullptr;   // NPAR x nchunks
     double *partials = nullptr;
+    void *members = nullptr; // AdjMember<R>[nmembers]
+    int4 *chunks = nullptr;
+    void *tables = nullptr;  // batch recompute: seg_cap tables of nmembers StepArgs
 };
 
 void free_adjoint(AdjointWork *w)
 {
     if (!w)
         return;
-    void *p[] = {w->ckpt, w->seg, w->lam, w->tmp, w->code, w->acc, w->partials};
+    void *p[] = {w->ckpt, w->seg, w->lam, w->tmp, w->code, w->acc, w->partials, w->members,
+                 w->chunks, w->tables};
     for (void *q : p)
         if (q)
             (void)hipFree(q);
@@ -394,70 +455,104 @@ void free_adjoint(AdjointWork *w)
 
 namespace {
 
-int ensure_work(kg_solver_s &h, int nckpt, int seglen)
+// Buffers for `total` elements per field, sized for nckpt checkpoints and
+// segments of seglen steps; reused while large enough.
+int ensure_work(AdjointWork *&w, size_t total, size_t es, int nckpt, int seglen, int nmembers,
+                int nchunks, size_t args_bytes, bool tables)
 {
-    AdjointWork *w = h.adj;
-    const size_t n = h.cells(), es = h.esize(), fb = n * es;
-    if (w && (w->ckpt_cap < nckpt || w->seg_cap < seglen)) {
+    const size_t fb = total * es;
+    if (w && (w->total != total || w->esize != es || w->ckpt_cap < nckpt || w->seg_cap < seglen ||
+              w->nmembers != nmembers || w->nchunks != nchunks || (tables && !w->tables))) {
         free_adjoint(w);
-        w = h.adj = nullptr;
+        w = nullptr;
     }
     if (w)
         return KG_OK;
     w = new AdjointWork();
-    w->n = n;
+    w->total = total;
     w->esize = es;
     w->ckpt_cap = nckpt;
     w->seg_cap = seglen;
+    w->nmembers = nmembers;
+    w->nchunks = nchunks;
     bool ok = hipMalloc(&w->ckpt, 3 * fb * (size_t)nckpt) == hipSuccess &&
               hipMalloc(&w->seg, 3 * fb * (size_t)(seglen + 1)) == hipSuccess &&
               hipMalloc(&w->lam, 6 * fb) == hipSuccess &&
               hipMalloc(&w->tmp, (NTEMP + 1) * fb) == hipSuccess &&
-              hipMalloc(&w->code, n) == hipSuccess &&
-              hipMalloc(&w->acc, (size_t)NPAR * ADJ_BLOCKS * sizeof(double)) == hipSuccess &&
-              hipMalloc(&w->partials, ADJ_BLOCKS * sizeof(double)) == hipSuccess;
+              hipMalloc(&w->code, total) == hipSuccess &&
+              hipMalloc(&w->acc, (size_t)NPAR * nchunks * sizeof(double)) == hipSuccess &&
+              hipMalloc(&w->partials, (size_t)nchunks * sizeof(double)) == hipSuccess &&
+              hipMalloc(&w->members, args_bytes) == hipSuccess &&
+              hipMalloc(&w->chunks, (size_t)nchunks * sizeof(int4)) == hipSuccess &&
+              (!tables || hipMalloc(&w->tables, (size_t)seglen * nmembers * sizeof(StepArgs<double>)) ==
+                              hipSuccess);
     if (!ok) {
         free_adjoint(w);
+        w = nullptr;
         return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the adjoint buffers failed");
     }
-    h.adj = w;
     return KG_OK;
 }
 
+// What the driver needs from a single grid or a batch.
 template <typename R>
-int adjoint_impl(kg_solver_s &h, int nsteps, int every, const void *target, kg_grads *out,
-                 void *d_T0, void *d_phi0)
+struct AdjJob {
+    hipStream_t stream;
+    size_t total;                       // elements per field buffer
+    std::vector<int> nx, ny;
+    std::vector<size_t> off;
+    std::vector<kg_params> params;
+    uint64_t s0;                        // noise counter of the first step
+    R *cur_phi, *cur_T, *cur_theta;     // current state buffers (updated by fwd)
+    std::function<int()> fwd;           // one step on the job's own buffers
+    // one recompute step i: state seg[i] -> seg[i+1] with step counter `step`
+    std::function<int(int i, R *const *in, R *const *out, uint32_t step)> recompute;
+};
+
+template <typename R>
+int adjoint_core(AdjJob<R> &job, AdjointWork *&work, int nsteps, int every, const R *target_host,
+                 std::vector<kg_grads> &grads, R *d_T0_dev_copy_host, R *d_phi0_host,
+                 bool tables)
 {
-    const size_t n = h.cells(), fb = h.field_bytes();
+    const int M = (int)job.nx.size();
+    const size_t total = job.total, es = sizeof(R), fb = total * es;
     const int nseg = (nsteps + every - 1) / every;
-    if (int rc = ensure_work(h, nseg, every))
+    std::vector<int4> chunks;
+    for (int m = 0; m < M; ++m) {
+        const size_t n = (size_t)job.nx[m] * job.ny[m];
+        for (size_t c = 0; c < n; c += ADJ_CHUNK)
+            chunks.push_back(make_int4(m, (int)c, (int)std::min((size_t)ADJ_CHUNK, n - c), 0));
+    }
+    const int nch = (int)chunks.size();
+    for (int k = 0; k < nch; ++k)
+        chunks[k].w = k;
+    if (int rc = ensure_work(work, total, es, nseg, every, M, nch, (size_t)M * sizeof(AdjMember<R>),
+                             tables))
         return rc;
-    AdjointWork &w = *h.adj;
-    const hipStream_t s = h.stream;
+    AdjointWork &w = *work;
+    const hipStream_t s = job.stream;
+    std::vector<AdjMember<R>> mem(M);
+    for (int m = 0; m < M; ++m) {
+        mem[m].a = make_args<R>(job.params[m], job.nx[m], job.ny[m]);
+        mem[m].off = job.off[m];
+        mem[m].tau = (R)job.params[m].tau;
+    }
+    KG_HIP(hipMemcpyAsync(w.members, mem.data(), M * sizeof(AdjMember<R>), hipMemcpyHostToDevice, s));
+    KG_HIP(hipMemcpyAsync(w.chunks, chunks.data(), nch * sizeof(int4), hipMemcpyHostToDevice, s));
+    KG_HIP(hipStreamSynchronize(s));
     auto st = [&](char *base, int k, int f) { return reinterpret_cast<R *>(base + (3 * (size_t)k + f) * fb); };
     R *tgt = reinterpret_cast<R *>(w.tmp + (size_t)NTEMP * fb);
-    KG_HIP(hipMemcpyAsync(tgt, target, fb, hipMemcpyHostToDevice, s));
-    const uint64_t s0 = h.steps;
-    StepArgs<R> base = make_args<R>(h.params, h.nx, h.ny);
+    KG_HIP(hipMemcpyAsync(tgt, target_host, fb, hipMemcpyHostToDevice, s));
 
-    // forward with checkpoints, on the handle's own buffers
+    // forward with checkpoints
     for (int k = 0; k < nseg; ++k) {
-        KG_HIP(hipMemcpyAsync(st(w.ckpt, k, 0), h.phi[h.cur], fb, hipMemcpyDeviceToDevice, s));
-        KG_HIP(hipMemcpyAsync(st(w.ckpt, k, 1), h.T[h.cur], fb, hipMemcpyDeviceToDevice, s));
-        KG_HIP(hipMemcpyAsync(st(w.ckpt, k, 2), h.theta, fb, hipMemcpyDeviceToDevice, s));
+        KG_HIP(hipMemcpyAsync(st(w.ckpt, k, 0), job.cur_phi, fb, hipMemcpyDeviceToDevice, s));
+        KG_HIP(hipMemcpyAsync(st(w.ckpt, k, 1), job.cur_T, fb, hipMemcpyDeviceToDevice, s));
+        KG_HIP(hipMemcpyAsync(st(w.ckpt, k, 2), job.cur_theta, fb, hipMemcpyDeviceToDevice, s));
         const int len = std::min(every, nsteps - k * every);
-        for (int i = 0; i < len; ++i) {
-            StepArgs<R> a = base;
-            a.phi_in = static_cast<const R *>(h.phi[h.cur]);
-            a.T_in = static_cast<const R *>(h.T[h.cur]);
-            a.phi_out = static_cast<R *>(h.phi[h.cur ^ 1]);
-            a.T_out = static_cast<R *>(h.T[h.cur ^ 1]);
-            a.theta = static_cast<R *>(h.theta);
-            a.step = (uint32_t)h.steps;
-            KG_HIP(launch_step_fused<R>(a, s));
-            h.cur ^= 1;
-            h.steps += 1;
-        }
+        for (int i = 0; i < len; ++i)
+            if (int rc = job.fwd())
+                return rc;
     }
 
     // loss and the seed adjoint
@@ -465,18 +560,19 @@ int adjoint_impl(kg_solver_s &h, int nsteps, int every, const void *target, kg_g
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 3; ++f)
             lam[b][f] = reinterpret_cast<R *>(w.lam + (3 * (size_t)b + f) * fb);
-    hipLaunchKernelGGL((k_adj_seed<R>), dim3(ADJ_BLOCKS), dim3(ADJ_THREADS), 0, s,
-                       static_cast<const R *>(h.phi[h.cur]), (const R *)tgt, n, lam[0][0],
-                       lam[0][1], lam[0][2], w.partials);
+    const AdjMember<R> *dmem = static_cast<const AdjMember<R> *>(w.members);
+    hipLaunchKernelGGL((k_adj_seed<R>), dim3(nch), dim3(ADJ_THREADS), 0, s, dmem, w.chunks,
+                       (const R *)job.cur_phi, (const R *)tgt, lam[0][0], lam[0][1], lam[0][2],
+                       w.partials);
     KG_HIP(hipGetLastError());
-    KG_HIP(hipMemsetAsync(w.acc, 0, (size_t)NPAR * ADJ_BLOCKS * sizeof(double), s));
+    KG_HIP(hipMemsetAsync(w.acc, 0, (size_t)NPAR * nch * sizeof(double), s));
 
     AdjTemps<R> t;
     R *tp = reinterpret_cast<R *>(w.tmp);
     R **fields[] = {&t.gx, &t.gy, &t.lap, &t.lapT, &t.th, &t.eps, &t.epsd, &t.cs, &t.sn,
                     &t.ls, &t.gEx, &t.gEy, &t.lphi_loc, &t.lT_loc, &t.lgx, &t.lgy, &t.lth_old};
     for (int f = 0; f < NTEMP; ++f)
-        *fields[f] = tp + (size_t)f * n;
+        *fields[f] = tp + (size_t)f * total;
     t.code = w.code;
 
     int lc = 0;
@@ -486,65 +582,175 @@ int adjoint_impl(kg_solver_s &h, int nsteps, int every, const void *target, kg_g
             KG_HIP(hipMemcpyAsync(st(w.seg, 0, f), st(w.ckpt, k, f), fb, hipMemcpyDeviceToDevice, s));
         for (int i = 0; i < len; ++i) {  // recompute the segment's states
             KG_HIP(hipMemcpyAsync(st(w.seg, i + 1, 2), st(w.seg, i, 2), fb, hipMemcpyDeviceToDevice, s));
-            StepArgs<R> a = base;
-            a.phi_in = st(w.seg, i, 0);
-            a.T_in = st(w.seg, i, 1);
-            a.phi_out = st(w.seg, i + 1, 0);
-            a.T_out = st(w.seg, i + 1, 1);
-            a.theta = st(w.seg, i + 1, 2);
-            a.step = (uint32_t)(s0 + (uint64_t)k * every + i);
-            KG_HIP(launch_step_fused<R>(a, s));
+            R *in[3] = {st(w.seg, i, 0), st(w.seg, i, 1), st(w.seg, i + 1, 2)};
+            R *out[3] = {st(w.seg, i + 1, 0), st(w.seg, i + 1, 1), st(w.seg, i + 1, 2)};
+            if (int rc = job.recompute(i, in, out, (uint32_t)(job.s0 + (uint64_t)k * every + i)))
+                return rc;
         }
         for (int i = len - 1; i >= 0; --i) {
-            StepArgs<R> a = base;
-            a.step = (uint32_t)(s0 + (uint64_t)k * every + i);
             AdjIO<R> io{st(w.seg, i, 0), st(w.seg, i, 1), st(w.seg, i, 2),
                         lam[lc][0], lam[lc][1], lam[lc][2],
                         lam[lc ^ 1][0], lam[lc ^ 1][1], lam[lc ^ 1][2]};
-            const hipError_t e = h.params.boundary == KG_ZERO_FLUX
-                                     ? reverse_step<R, 1>(a, io, t, (R)h.params.tau, w.acc, s)
-                                     : reverse_step<R, 0>(a, io, t, (R)h.params.tau, w.acc, s);
+            const hipError_t e = reverse_step<R>(dmem, w.chunks, nch, io, t, w.acc,
+                                                 (uint32_t)(job.s0 + (uint64_t)k * every + i), s);
             if (e != hipSuccess)
                 return hip_fail(e, "adjoint step launch");
             lc ^= 1;
         }
     }
 
-    std::vector<double> acc((size_t)NPAR * ADJ_BLOCKS), parts(ADJ_BLOCKS);
+    std::vector<double> acc((size_t)NPAR * nch), parts(nch);
     KG_HIP(hipMemcpyAsync(acc.data(), w.acc, acc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     KG_HIP(hipMemcpyAsync(parts.data(), w.partials, parts.size() * sizeof(double),
                           hipMemcpyDeviceToHost, s));
-    if (d_T0)
-        KG_HIP(hipMemcpyAsync(d_T0, lam[lc][1], fb, hipMemcpyDeviceToHost, s));
-    if (d_phi0)
-        KG_HIP(hipMemcpyAsync(d_phi0, lam[lc][0], fb, hipMemcpyDeviceToHost, s));
+    if (d_T0_dev_copy_host)
+        KG_HIP(hipMemcpyAsync(d_T0_dev_copy_host, lam[lc][1], fb, hipMemcpyDeviceToHost, s));
+    if (d_phi0_host)
+        KG_HIP(hipMemcpyAsync(d_phi0_host, lam[lc][0], fb, hipMemcpyDeviceToHost, s));
     KG_HIP(hipStreamSynchronize(s));
-    double g[NPAR];
-    for (int p = 0; p < NPAR; ++p) {
-        double v = 0.0;
-        for (int b = 0; b < ADJ_BLOCKS; ++b)
-            v += acc[(size_t)p * ADJ_BLOCKS + b];
-        g[p] = v;
+    grads.assign(M, kg_grads{});
+    for (int k = 0; k < nch; ++k) {
+        kg_grads &g = grads[chunks[k].x];
+        double *q = &g.d_tau;
+        const int map[NPAR] = {P_TAU, P_EPS_BAR, P_K, P_DELTA, P_THETA0, P_ALPHA, P_GAMMA, P_TEQ,
+                               P_NOISE};
+        for (int p = 0; p < NPAR; ++p)
+            q[p] += acc[(size_t)map[p] * nch + k];
+        g.loss += parts[k];
     }
-    double loss = 0.0;
-    for (double v : parts)
-        loss += v;
-    out->loss = loss;
-    out->d_tau = g[P_TAU];
-    out->d_eps_bar = g[P_EPS_BAR];
-    out->d_K = g[P_K];
-    out->d_delta = g[P_DELTA];
-    out->d_theta0 = g[P_THETA0];
-    out->d_alpha = g[P_ALPHA];
-    out->d_gamma = g[P_GAMMA];
-    out->d_teq = g[P_TEQ];
-    out->d_noise_a = g[P_NOISE];
+    return KG_OK;
+}
+
+template <typename R>
+int adjoint_single(kg_solver_s &h, int nsteps, int every, const void *target, kg_grads *out,
+                   void *d_T0, void *d_phi0)
+{
+    AdjJob<R> job;
+    job.stream = h.stream;
+    job.total = h.cells();
+    job.nx = {h.nx};
+    job.ny = {h.ny};
+    job.off = {0};
+    job.params = {h.params};
+    job.s0 = h.steps;
+    job.cur_phi = static_cast<R *>(h.phi[h.cur]);
+    job.cur_T = static_cast<R *>(h.T[h.cur]);
+    job.cur_theta = static_cast<R *>(h.theta);
+    const StepArgs<R> base = make_args<R>(h.params, h.nx, h.ny);
+    job.fwd = [&]() -> int {
+        StepArgs<R> a = base;
+        a.phi_in = static_cast<const R *>(h.phi[h.cur]);
+        a.T_in = static_cast<const R *>(h.T[h.cur]);
+        a.phi_out = static_cast<R *>(h.phi[h.cur ^ 1]);
+        a.T_out = static_cast<R *>(h.T[h.cur ^ 1]);
+        a.theta = static_cast<R *>(h.theta);
+        a.step = (uint32_t)h.steps;
+        KG_HIP(launch_step_fused<R>(a, h.stream));
+        h.cur ^= 1;
+        h.steps += 1;
+        job.cur_phi = static_cast<R *>(h.phi[h.cur]);
+        job.cur_T = static_cast<R *>(h.T[h.cur]);
+        return KG_OK;
+    };
+    job.recompute = [&](int, R *const *in, R *const *o, uint32_t step) -> int {
+        StepArgs<R> a = base;
+        a.phi_in = in[0];
+        a.T_in = in[1];
+        a.phi_out = o[0];
+        a.T_out = o[1];
+        a.theta = o[2];
+        a.step = step;
+        KG_HIP(launch_step_fused<R>(a, h.stream));
+        return KG_OK;
+    };
+    std::vector<kg_grads> g;
+    if (int rc = adjoint_core<R>(job, h.adj, nsteps, every, static_cast<const R *>(target), g,
+                                 static_cast<R *>(d_T0), static_cast<R *>(d_phi0), false))
+        return rc;
+    *out = g[0];
+    return KG_OK;
+}
+
+template <typename R>
+int adjoint_batch(kg_batch_s &b, int nsteps, int every, const void *const *targets, kg_grads *out,
+                  void *const *d_T0)
+{
+    const int M = b.count;
+    AdjJob<R> job;
+    job.stream = b.stream;
+    job.total = b.total;
+    job.nx = b.nx;
+    job.ny = b.ny;
+    job.off = b.off;
+    job.params = b.params;
+    job.s0 = b.steps;
+    job.cur_phi = static_cast<R *>(b.phi[b.cur()]);
+    job.cur_T = static_cast<R *>(b.T[b.cur()]);
+    job.cur_theta = static_cast<R *>(b.theta);
+    job.fwd = [&]() -> int {
+        if (int rc = kg_batch_step(&b, 1))
+            return rc;
+        job.cur_phi = static_cast<R *>(b.phi[b.cur()]);
+        job.cur_T = static_cast<R *>(b.T[b.cur()]);
+        return KG_OK;
+    };
+    std::vector<char> built(every, 0);
+    job.recompute = [&](int i, R *const *in, R *const *o, uint32_t step) -> int {
+        StepArgs<R> *table = static_cast<StepArgs<R> *>(b.adj->tables) + (size_t)i * M;
+        if (!built[i]) {
+            std::vector<StepArgs<R>> v(M);
+            for (int m = 0; m < M; ++m) {
+                StepArgs<R> a = make_args<R>(b.params[m], b.nx[m], b.ny[m]);
+                a.phi_in = in[0] + b.off[m];
+                a.T_in = in[1] + b.off[m];
+                a.phi_out = o[0] + b.off[m];
+                a.T_out = o[1] + b.off[m];
+                a.theta = o[2] + b.off[m];
+                v[m] = a;
+            }
+            KG_HIP(hipMemcpy(table, v.data(), M * sizeof(StepArgs<R>), hipMemcpyHostToDevice));
+            built[i] = 1;
+        }
+        KG_HIP(launch_step_batch_direct<R>(table, b.tiles, b.ntiles, step, b.stream));
+        return KG_OK;
+    };
+    std::vector<R> tgt(b.total, (R)0), dT(d_T0 ? b.total : 0);
+    for (int m = 0; m < M; ++m)
+        std::memcpy(tgt.data() + b.off[m], targets[m], b.cells(m) * sizeof(R));
+    std::vector<kg_grads> g;
+    if (int rc = adjoint_core<R>(job, b.adj, nsteps, every, tgt.data(), g,
+                                 d_T0 ? dT.data() : nullptr, nullptr, true))
+        return rc;
+    for (int m = 0; m < M; ++m) {
+        out[m] = g[m];
+        if (d_T0 && d_T0[m])
+            std::memcpy(d_T0[m], dT.data() + b.off[m], b.cells(m) * sizeof(R));
+    }
     return KG_OK;
 }
 
 }  // namespace
 
 }  // namespace kg
+
+extern "C" int kg_batch_adjoint(kg_batch b, int nsteps, int checkpoint_every,
+                                const void *const *targets, kg_grads *out, void *const *d_T0)
+{
+    if (!b)
+        return kg::fail(KG_ERR_INVALID_ARGUMENT, "null batch");
+    if (!targets || !out)
+        return kg::fail(KG_ERR_INVALID_ARGUMENT, "null targets or output");
+    for (int m = 0; m < b->count; ++m)
+        if (!targets[m])
+            return kg::fail(KG_ERR_INVALID_ARGUMENT, "null target of a member");
+    if (nsteps < 1 || checkpoint_every < 1)
+        return kg::fail(KG_ERR_INVALID_ARGUMENT, "nsteps and checkpoint_every must be >= 1");
+    KG_HIP(hipSetDevice(b->device));
+    const int every = std::min(checkpoint_every, nsteps);
+    return b->precision == KG_F64
+               ? kg::adjoint_batch<double>(*b, nsteps, every, targets, out, d_T0)
+               : kg::adjoint_batch<float>(*b, nsteps, every, targets, out, d_T0);
+}
 
 extern "C" int kg_adjoint(kg_handle h, int nsteps, int checkpoint_every, const void *target,
                           size_t bytes, kg_grads *out, void *d_T0, void *d_phi0)
@@ -560,6 +766,6 @@ extern "C" int kg_adjoint(kg_handle h, int nsteps, int checkpoint_every, const v
     KG_HIP(hipSetDevice(h->device));
     const int every = std::min(checkpoint_every, nsteps);
     return h->precision == KG_F64
-               ? kg::adjoint_impl<double>(*h, nsteps, every, target, out, d_T0, d_phi0)
-               : kg::adjoint_impl<float>(*h, nsteps, every, target, out, d_T0, d_phi0);
+               ? kg::adjoint_single<double>(*h, nsteps, every, target, out, d_T0, d_phi0)
+               : kg::adjoint_single<float>(*h, nsteps, every, target, out, d_T0, d_phi0);
 }

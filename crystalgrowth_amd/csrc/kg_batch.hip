@@ -23,38 +23,6 @@
 using kg::fail;
 using kg::hip_fail;
 
-struct kg_batch_s {
-    int count = 0, precision = KG_F32;
-    std::vector<int> nx, ny;
-    std::vector<kg_params> params;
-    std::vector<size_t> off;   // element offset of each member in the field buffers
-    size_t total = 0;          // elements per field buffer
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    void *phi[2] = {nullptr, nullptr};
-    void *T[2] = {nullptr, nullptr};
-    void *theta = nullptr;
-    void *members = nullptr;   // StepArgs<R>[count]
-    int4 *tiles = nullptr;
-    int ntiles = 0;
-    uint32_t *step_base = nullptr;
-    double *partials = nullptr;
-    kg::ReduceOut *red = nullptr;
-    int *disks = nullptr;
-    int disks_cap = 0;
-    uint64_t steps = 0;        // steps since the last reset of all members
-    uint64_t base_host = 0;    // value of *step_base
-    uint32_t base_tmp = 0;     // host source of step_base uploads
-    int mode = KG_BATCH_LAUNCH, graph_steps = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-
-    size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
-    size_t cells(int m) const { return (size_t)nx[m] * (size_t)ny[m]; }
-    int cur() const { return (int)(steps & 1u); }
-    char *field(void *base, int m) const { return static_cast<char *>(base) + off[m] * esize(); }
-};
 
 namespace {
 
@@ -85,6 +53,8 @@ void free_batch(kg_batch_s *b)
     if (b->stream)
         (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
+    kg::free_adjoint(b->adj);
+    b->adj = nullptr;
     void *bufs[] = {b->phi[0], b->phi[1], b->T[0], b->T[1], b->theta, b->members,
                     b->tiles, b->step_base, b->partials, b->red, b->disks};
     for (void *p : bufs)
@@ -333,6 +303,40 @@ int kg_batch_destroy(kg_batch b)
     if (!b)
         return fail(KG_ERR_INVALID_ARGUMENT, "null batch");
     free_batch(b);
+    return KG_OK;
+}
+
+int kg_batch_set_params(kg_batch b, int member, const kg_params *p)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    if (int rc = kg_validate(b->nx[member], b->ny[member], p))
+        return rc;
+    KG_HIP(hipSetDevice(b->device));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    b->params[member] = *p;
+    hipError_t e;
+    if (b->precision == KG_F64) {
+        const auto v = member_args<double>(*b);
+        e = hipMemcpy(static_cast<kg::StepArgs<double> *>(b->members) + member, &v[member],
+                      sizeof(v[member]), hipMemcpyHostToDevice);
+    } else {
+        const auto v = member_args<float>(*b);
+        e = hipMemcpy(static_cast<kg::StepArgs<float> *>(b->members) + member, &v[member],
+                      sizeof(v[member]), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "member parameter upload");
+    return KG_OK;
+}
+
+int kg_batch_get_params(kg_batch b, int member, kg_params *p)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    if (!p)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null params");
+    *p = b->params[member];
     return KG_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/kobayashi_gpu.h"
 #include "kg_internal.h"
@@ -37,6 +38,41 @@ struct kg_solver_s {
     size_t cells() const { return (size_t)nx * (size_t)ny; }
     size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
     size_t field_bytes() const { return cells() * esize(); }
+};
+
+// A batch handle (kg_batch): every member's fields in one allocation per field.
+struct kg_batch_s {
+    int count = 0, precision = KG_F32;
+    std::vector<int> nx, ny;
+    std::vector<kg_params> params;
+    std::vector<size_t> off;   // element offset of each member in the field buffers
+    size_t total = 0;          // elements per field buffer
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    void *phi[2] = {nullptr, nullptr};
+    void *T[2] = {nullptr, nullptr};
+    void *theta = nullptr;
+    void *members = nullptr;   // StepArgs<R>[count]
+    int4 *tiles = nullptr;
+    int ntiles = 0;
+    uint32_t *step_base = nullptr;
+    double *partials = nullptr;
+    kg::ReduceOut *red = nullptr;
+    int *disks = nullptr;
+    int disks_cap = 0;
+    uint64_t steps = 0;        // steps since the last reset of all members
+    uint64_t base_host = 0;    // value of *step_base
+    uint32_t base_tmp = 0;     // host source of step_base uploads
+    int mode = KG_BATCH_LAUNCH, graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    kg::AdjointWork *adj = nullptr;  // batched adjoint buffers, allocated on first use
+
+    size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
+    size_t cells(int m) const { return (size_t)nx[m] * (size_t)ny[m]; }
+    int cur() const { return (int)(steps & 1u); }
+    char *field(void *base, int m) const { return static_cast<char *>(base) + off[m] * esize(); }
 };
 
 namespace kg {

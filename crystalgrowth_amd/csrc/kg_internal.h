@@ -78,6 +78,10 @@ int fused_tile_y();
 template <typename R>
 hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
                              const uint32_t *step_base, uint32_t step_add, hipStream_t s);
+// Batched step with the members' pointers as given and an absolute step.
+template <typename R>
+hipError_t launch_step_batch_direct(const StepArgs<R> *members, const int4 *tiles, int ntiles,
+                                    uint32_t step, hipStream_t s);
 hipError_t launch_advance(uint32_t *step_base, uint32_t n, hipStream_t s);
 
 }  // namespace kg

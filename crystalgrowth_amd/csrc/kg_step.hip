@@ -36,14 +36,16 @@ namespace kg {
 namespace {
 
 constexpr int TX = 64;        // output tile width (x, contiguous)
-constexpr int TY = 16;        // output tile height
-constexpr int NT = 256;       // threads per block (4 wave64), one 4-cell row group each
+constexpr int TY = 30;        // output tile height
+constexpr int NT = 512;       // threads per block (8 wave64)
+constexpr int NG = TX / 4;    // 4-cell groups per row
 constexpr int LW = TX + 8;    // LDS row: columns x0-4 .. x0+67 (16-byte aligned)
 constexpr int PH = TY + 4;    // phi rows y0-2 .. y0+TY+1
-constexpr int QH = TY + 2;    // T and pass-1 rows y0-1 .. y0+TY
-constexpr int DW = TX + 2;    // pass-1 cells per row: x0-1 .. x0+64 (LDS columns 3 .. 68)
+constexpr int QH = TY + 2;    // T, theta and pass-1 rows y0-1 .. y0+TY
 constexpr int NXCD = 8;       // MI355X: workgroups go round-robin over 8 XCDs
-static_assert(NT == (TX / 4) * TY, "one thread per 4-cell row group");
+static_assert(NG * QH == NT, "pass 1: one 4-cell group per thread");
+static_assert(NG * TY <= NT, "pass 2: one 4-cell group per thread");
+static_assert(2 * QH <= 64, "pass 1 halo columns: one wave");
 
 template <typename R> struct Consts;
 template <> struct Consts<float> {
@@ -77,6 +79,28 @@ __device__ __forceinline__ void store4(double *p, const double v[4])
 {
     reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
     reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
+
+// all four exactly +0 / all four exactly 1 (bit patterns, so -0 is not flat)
+__device__ __forceinline__ void flat4(const float v[4], int &all0, int &all1)
+{
+    const uint32_t a = __float_as_uint(v[0]), b = __float_as_uint(v[1]);
+    const uint32_t c = __float_as_uint(v[2]), d = __float_as_uint(v[3]);
+    const uint32_t o = a | b | c | d, n = a & b & c & d;
+    all0 &= (o == 0u);
+    all1 &= (o == 0x3f800000u) & (n == 0x3f800000u);
+}
+__device__ __forceinline__ void flat4(const double v[4], int &all0, int &all1)
+{
+    for (int e = 0; e < 4; ++e) {
+        all0 &= (__double_as_longlong(v[e]) == 0ll);
+        all1 &= (v[e] == 1.0);
+    }
+}
+template <typename R>
+__device__ __forceinline__ int finite4(const R v[4])
+{
+    return (int)isfinite(v[0]) & (int)isfinite(v[1]) & (int)isfinite(v[2]) & (int)isfinite(v[3]);
 }
 
 template <int BC>
@@ -146,7 +170,9 @@ __device__ __forceinline__ int xcd_tile(int per_xcd)
 }
 
 // LDS of one tile: phi (halo 2), T (halo 1) and the pass-1 values that pass 2
-// reads from neighbours, each row covering columns x0-4 .. x0+67.
+// reads from neighbours, each row covering columns x0-4 .. x0+67.  The old
+// theta of non-flat tiles is staged in B: every pass-1 cell reads its own
+// theta slot before it writes its own B there.
 template <typename R>
 struct TileSmem {
     R P[PH * LW];
@@ -156,7 +182,29 @@ struct TileSmem {
     R B[QH * LW];
 };
 
-// One explicit-Euler step of the 64 x 16 output tile at (x0, y0).
+// pass-1 values of one cell from its phi neighbourhood (rows m, z, p of sP
+// around column c) and its old theta.  Returns e^2, eps eps' gx, eps eps' gy
+// and the theta the step leaves (fired: true when a branch rewrote it).
+template <typename R>
+__device__ __forceinline__ void pass1_cell(const StepArgs<R> &a, const R *m, const R *z,
+                                           const R *p, int c, R th_old, R &e2, R &A, R &B,
+                                           R &th, bool &fired)
+{
+    const R gx = (z[c + 1] - z[c - 1]) * a.inv_dx;
+    const R gy = (p[c] - m[c]) * a.inv_dy;
+    fired = theta_branch(gx, gy, th);
+    if (!fired)
+        th = th_old;
+    R sn, cs;
+    kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
+    const R eps = a.eps_bar * ((R)1.0 + a.delta * cs);
+    const R f = eps * (a.neg_ebar_j_delta * sn);
+    e2 = eps * eps;
+    A = f * gx;
+    B = f * gy;
+}
+
+// One explicit-Euler step of the 64 x 30 output tile at (x0, y0).
 template <typename R, int BC>
 __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, TileSmem<R> &sm)
 {
@@ -172,27 +220,21 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     int all0 = 1, all1 = 1, finite = 1;
     const bool interior = x0 >= 4 && x0 + TX + 4 <= nx && y0 >= 2 && y0 + TY + 2 <= ny &&
                           (nx & 3) == 0;
+    constexpr int V = LW / 4;
     if (interior) {
-        constexpr int V = LW / 4;
         for (int k = tid; k < PH * V; k += NT) {
             const int r = k / V, v = k - r * V;
             R q[4];
             load4(a.phi_in + (size_t)(y0 - 2 + r) * nx + (x0 - 4 + 4 * v), q);
             store4(&sP[r * LW + 4 * v], q);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                all0 &= is_pos_zero(q[e]);
-                all1 &= (q[e] == (R)1.0);
-            }
+            flat4(q, all0, all1);
         }
         for (int k = tid; k < QH * V; k += NT) {
             const int r = k / V, v = k - r * V;
             R q[4];
             load4(a.T_in + (size_t)(y0 - 1 + r) * nx + (x0 - 4 + 4 * v), q);
             store4(&sT[r * LW + 4 * v], q);
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                finite &= isfinite(q[e]) ? 1 : 0;
+            finite &= finite4(q);
         }
     } else {
         for (int k = tid; k < PH * LW; k += NT) {
@@ -215,49 +257,101 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     const int flat1 = __syncthreads_and(all1 & finite);
     const bool flat = flat0 || flat1;
 
-    // ---- pass 1 on the tile plus a 1-cell halo ----------------------------
     if (!flat) {
-        for (int k = tid; k < QH * DW; k += NT) {
-            const int r = k / DW, c = k - r * DW;
-            const int ux = x0 - 1 + c, uy = y0 - 1 + r;  // unmapped global coords
-            int cx, cy, lc = c + 3, lr = r;              // LDS column / pass-1 row used
+        // ---- stage the old theta of the pass-1 cells into B ---------------
+        if (interior) {
+            for (int k = tid; k < QH * V; k += NT) {
+                const int r = k / V, v = k - r * V;
+                R q[4];
+                load4(a.theta + (size_t)(y0 - 1 + r) * nx + (x0 - 4 + 4 * v), q);
+                store4(&sB[r * LW + 4 * v], q);
+            }
+        } else {
+            for (int k = tid; k < QH * LW; k += NT) {
+                const int r = k / LW, c = k - r * LW;
+                sB[k] = a.theta[(size_t)map_index<BC>(y0 - 1 + r, ny) * nx +
+                                map_index<BC>(x0 - 4 + c, nx)];
+            }
+        }
+        __syncthreads();
+
+        // ---- pass 1: one 4-cell group per thread (columns x0 .. x0+63) ----
+        {
+            const int g = tid % NG, r = tid / NG;       // pass-1 row r: global y0-1+r
+            const int kc = 4 * g + 4;                   // LDS column of the first cell
+            const int ux = x0 + 4 * g, uy = y0 - 1 + r;
+            R e2[4], A[4], B[4], th[4], thold[4];
+            bool fired[4];
+            load4(&sB[r * LW + kc], thold);
+            // zero-flux ghost rows/columns take the values of the cell they
+            // clamp to; periodic halos already hold wrapped values
+            const bool shifted = BC == 1 && (uy < 0 || uy >= ny || ux + 3 >= nx);
+            if (!shifted) {
+                R m[6], z[6], p[6];
+                const R *pm = &sP[r * LW], *pz = &sP[(r + 1) * LW], *pp = &sP[(r + 2) * LW];
+                z[0] = pz[kc - 1];
+                load4(pz + kc, z + 1);
+                z[5] = pz[kc + 4];
+                load4(pm + kc, m + 1);
+                load4(pp + kc, p + 1);
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    pass1_cell(a, m, z, p, e + 1, thold[e], e2[e], A[e], B[e], th[e], fired[e]);
+            } else {
+                const int cy = uy < 0 ? 0 : (uy >= ny ? ny - 1 : uy);
+                const int lr = r + (cy - uy);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int cx = ux + e >= nx ? nx - 1 : ux + e;
+                    const int lc = kc + e + (cx - (ux + e));
+                    pass1_cell(a, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
+                               thold[e], e2[e], A[e], B[e], th[e], fired[e]);
+                }
+            }
+            store4(&sE2[r * LW + kc], e2);
+            store4(&sA[r * LW + kc], A);
+            store4(&sB[r * LW + kc], B);
+            // theta of owned cells: rewrite the group (kept values unchanged)
+            // when any of its cells fired
+            if (r >= 1 && r <= TY && uy < ny && (fired[0] | fired[1] | fired[2] | fired[3])) {
+                R *dst = a.theta + (size_t)uy * nx + ux;
+                if (ux + 3 < nx && (nx & 3) == 0) {
+                    store4(dst, th);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (ux + e < nx)
+                            dst[e] = th[e];
+                }
+            }
+        }
+        // ---- pass 1, halo columns x0-1 and x0+64 (one wave) ---------------
+        if (tid < 2 * QH) {
+            const int r = tid >> 1, side = tid & 1;
+            const int lcol = side ? TX + 4 : 3;          // LDS column
+            const int ux = x0 - 4 + lcol, uy = y0 - 1 + r;
+            int lc = lcol, lr = r;
             if (BC == 1) {
-                // zero-flux: a ghost cell carries the pass-1 values of the
-                // boundary cell it clamps to (pass 2 clamps its indices too)
-                cx = ux < 0 ? 0 : (ux >= nx ? nx - 1 : ux);
-                cy = uy < 0 ? 0 : (uy >= ny ? ny - 1 : uy);
+                const int cx = ux < 0 ? 0 : (ux >= nx ? nx - 1 : ux);
+                const int cy = uy < 0 ? 0 : (uy >= ny ? ny - 1 : uy);
                 lc += cx - ux;
                 lr += cy - uy;
-            } else {
-                cx = map_index<0>(ux, nx);
-                cy = map_index<0>(uy, ny);
             }
-            const R *pz = &sP[(lr + 1) * LW];
-            const R gx = (pz[lc + 1] - pz[lc - 1]) * a.inv_dx;
-            const R gy = (sP[(lr + 2) * LW + lc] - sP[lr * LW + lc]) * a.inv_dy;
-            const size_t gidx = (size_t)cy * nx + cx;
-            R th;
-            if (!theta_branch(gx, gy, th)) {
-                th = a.theta[gidx];
-            } else if (r >= 1 && r <= TY && c >= 1 && c <= TX && ux < nx && uy < ny) {
-                a.theta[gidx] = th;
-            }
-            R sn, cs;
-            kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
-            const R eps = a.eps_bar * ((R)1.0 + a.delta * cs);
-            const R f = eps * (a.neg_ebar_j_delta * sn);
-            const int o = r * LW + c + 3;
-            sE2[o] = eps * eps;
-            sA[o] = f * gx;
-            sB[o] = f * gy;
+            R e2, A, B, th;
+            bool fired;
+            pass1_cell(a, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
+                       sB[r * LW + lcol], e2, A, B, th, fired);
+            sE2[r * LW + lcol] = e2;
+            sA[r * LW + lcol] = A;
+            sB[r * LW + lcol] = B;
         }
         __syncthreads();
     }
 
     // ---- pass 2: one row of 4 cells per thread ----------------------------
-    const int g = tid & (TX / 4 - 1), row = tid / (TX / 4);
+    const int g = tid % NG, row = tid / NG;
     const int i0 = x0 + 4 * g, j = y0 + row;
-    if (j >= ny || i0 >= nx)
+    if (row >= TY || j >= ny || i0 >= nx)
         return;
     const int kc = 4 * g + 4;  // LDS column of the first cell
     // 6-wide windows: index 0 = column i0-1, 1..4 = the cells, 5 = i0+4
@@ -351,8 +445,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     }
 }
 
+// FP32: 3 blocks of 8 waves per CU fit the LDS (3 x 46.9 KB); the register
+// allocator is asked for the matching 6 waves per SIMD.  FP64 needs twice the
+// LDS and registers and runs one block per CU.
+#ifndef KG_WAVES_PER_EU
+#define KG_WAVES_PER_EU 6
+#endif
+template <typename R> struct Occupancy;
+template <> struct Occupancy<float> { static constexpr int waves = KG_WAVES_PER_EU; };
+template <> struct Occupancy<double> { static constexpr int waves = 1; };
+
 template <typename R, int BC>
-__global__ __launch_bounds__(NT) void k_step_fused(StepArgs<R> a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a)
 {
     __shared__ __align__(16) TileSmem<R> sm;
     const int tile = xcd_tile(a.per_xcd);
@@ -369,7 +473,7 @@ __global__ __launch_bounds__(NT) void k_step_fused(StepArgs<R> a)
 // direction (a batch always restarts at step 0 in buffer 0), so a captured
 // graph replays correctly after its k_advance node moves *step_base on.
 template <typename R>
-__global__ __launch_bounds__(NT) void k_step_batch(const StepArgs<R> *__restrict__ members,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_batch(const StepArgs<R> *__restrict__ members,
                                                    const int4 *__restrict__ tiles, int ntiles,
                                                    int per_xcd,
                                                    const uint32_t *__restrict__ step_base,
@@ -389,6 +493,26 @@ __global__ __launch_bounds__(NT) void k_step_batch(const StepArgs<R> *__restrict
         a.phi_out = const_cast<R *>(pi);
         a.T_out = const_cast<R *>(ti);
     }
+    if (a.boundary == 1)
+        step_tile<R, 1>(a, d.y, d.z, sm);
+    else
+        step_tile<R, 0>(a, d.y, d.z, sm);
+}
+
+// Batched step with the members' pointers used as given (no ping-pong) and an
+// absolute step counter: the adjoint's recompute of a stored segment.
+template <typename R>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_batch_direct(
+    const StepArgs<R> *__restrict__ members, const int4 *__restrict__ tiles, int ntiles,
+    int per_xcd, uint32_t step)
+{
+    __shared__ __align__(16) TileSmem<R> sm;
+    const int t = xcd_tile(per_xcd);
+    if (t >= ntiles)
+        return;
+    const int4 d = tiles[t];
+    StepArgs<R> a = members[d.x];
+    a.step = step;
     if (a.boundary == 1)
         step_tile<R, 1>(a, d.y, d.z, sm);
     else
@@ -673,6 +797,16 @@ hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int 
     return hipGetLastError();
 }
 
+template <typename R>
+hipError_t launch_step_batch_direct(const StepArgs<R> *members, const int4 *tiles, int ntiles,
+                                    uint32_t step, hipStream_t s)
+{
+    const int per = (ntiles + NXCD - 1) / NXCD;
+    hipLaunchKernelGGL((k_step_batch_direct<R>), dim3(per * NXCD), dim3(NT), 0, s, members, tiles,
+                       ntiles, per, step);
+    return hipGetLastError();
+}
+
 hipError_t launch_advance(uint32_t *step_base, uint32_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, s, step_base, n);
@@ -728,6 +862,8 @@ hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba,
     template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t);                  \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
                                              const uint32_t *, uint32_t, hipStream_t);           \
+    template hipError_t launch_step_batch_direct<R>(const StepArgs<R> *, const int4 *, int,      \
+                                                    uint32_t, hipStream_t);                      \
     template hipError_t launch_step_two_pass<R>(const StepArgs<R> &, const PassTemps<R> &,       \
                                                 hipStream_t);                                    \
     template hipError_t launch_init<R>(R *, R *, R *, int, int, int, int, int, const int *, int, \

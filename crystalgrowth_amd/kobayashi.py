@@ -312,6 +312,29 @@ class Batch:
         check(lib().kg_batch_reduce(self._h, member, ctypes.byref(s)))
         return s.as_dict()
 
+    def set_params(self, member: int, p: Params) -> None:
+        check(lib().kg_batch_set_params(self._h, member, ctypes.byref(p)))
+
+    def params(self, member: int) -> Params:
+        p = Params()
+        check(lib().kg_batch_get_params(self._h, member, ctypes.byref(p)))
+        return p
+
+    def adjoint(self, nsteps: int, targets, checkpoint_every: int = 50, want_T0: bool = False):
+        """kg_batch_adjoint: per-member (grads dict, dL/dT0 or None) from the
+        current state; the batch is left at the final forward state."""
+        n = len(self.sizes)
+        tg = []
+        for m, t in enumerate(targets):
+            nx, ny = self.sizes[m]
+            tg.append(np.ascontiguousarray(t, self.dtype).reshape(ny, nx))
+        tptr = (ctypes.c_void_p * n)(*[t.ctypes.data for t in tg])
+        dT = [np.empty((ny, nx), self.dtype) for nx, ny in self.sizes] if want_T0 else None
+        dptr = (ctypes.c_void_p * n)(*[d.ctypes.data for d in dT]) if want_T0 else None
+        out = (Grads * n)()
+        check(lib().kg_batch_adjoint(self._h, int(nsteps), int(checkpoint_every), tptr, out, dptr))
+        return [g.as_dict() for g in out], dT
+
     def set_mode(self, mode: str = "launch", graph_steps: int = 0) -> None:
         m = {"launch": KG_BATCH_LAUNCH, "graph": KG_BATCH_GRAPH}[mode]
         check(lib().kg_batch_set_mode(self._h, m, int(graph_steps)))

@@ -8,8 +8,9 @@ they agree on shape:
   radius_profile   distance from a nucleus centre to the phi = 1/2 front along
                    `n` directions (ray marching with linear interpolation)
   tip_radius       largest front distance (the dendrite tips)
-  arm_count        peaks of the radius profile that stand out by `prominence`
-                   (relative to the mean radius) -- the number of primary arms
+  arm_count        peaks of the radius profile whose prominence is at least
+                   `prominence` x (max - min radius) -- the number of primary arms
+                   (side branches of grown dendrites stand out far less)
   tip_velocity     growth rate of tip_radius between two snapshots
 """
 from __future__ import annotations
@@ -61,13 +62,14 @@ def tip_radius(phi: np.ndarray, cx: float, cy: float, n: int = 720) -> float:
 
 
 def arm_count(phi: np.ndarray, cx: float, cy: float, n: int = 720,
-              prominence: float = 0.15) -> int:
+              prominence: float = 0.3, min_relief: float = 0.15) -> int:
     """Number of circular peaks of the front radius whose height above the
-    neighbouring minima exceeds `prominence` x mean radius."""
+    neighbouring minima exceeds `prominence` x (max - min radius); a profile
+    whose relief (max - min) is below `min_relief` x mean radius has no arms."""
     r = radius_profile(phi, cx, cy, n)
-    if r.max() <= 0:
+    if r.max() <= 0 or r.max() - r.min() < min_relief * r.mean():
         return 0
-    thr = prominence * r.mean()
+    thr = prominence * (r.max() - r.min())
     # circular peak finding with a plateau-safe comparison
     peaks = []
     for k in range(n):

@@ -191,6 +191,9 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
                     const kg_params *params, kg_batch *out);
 int kg_batch_destroy(kg_batch b);
 int kg_batch_count(kg_batch b, int *count);
+/* Parameters of one member (takes effect at the next step; fields unchanged). */
+int kg_batch_set_params(kg_batch b, int member, const kg_params *p);
+int kg_batch_get_params(kg_batch b, int member, kg_params *p);
 /* member < 0 applies to every member (disk centred at (nx/2, ny/2) when cx < 0) */
 int kg_batch_reset_disk(kg_batch b, int member, int cx, int cy, double r);
 int kg_batch_reset_seeds(kg_batch b, int member, int count, double r, int margin, uint64_t seed);
@@ -210,6 +213,12 @@ int kg_batch_step(kg_batch b, int nsteps);
 int kg_batch_step_timed(kg_batch b, int nsteps, double *ms);
 int kg_batch_synchronize(kg_batch b);
 int kg_batch_get_step_count(kg_batch b, uint64_t *steps);
+/* kg_adjoint for every member at once: targets[m] and d_T0[m] (optional;
+ * the array or an entry may be NULL) are host fields of member m; out[m]
+ * receives member m's loss and gradients.  One launch per kernel covers all
+ * members; afterwards the batch holds the final forward state. */
+int kg_batch_adjoint(kg_batch b, int nsteps, int checkpoint_every, const void *const *targets,
+                     kg_grads *out, void *const *d_T0);
 
 #ifdef __cplusplus
 }
