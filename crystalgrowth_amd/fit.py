@@ -23,7 +23,7 @@ from . import configs
 from .kobayashi import Batch, Solver
 
 NAMES = ("delta", "K", "theta0")
-LR = {"delta": 2e-3, "K": 0.03, "theta0": 0.03}
+LR = {"delta": 1e-3, "K": 0.03, "theta0": 0.03}
 BOUNDS = {"delta": (0.0, 0.2), "K": (0.2, 4.0), "theta0": (-np.pi, np.pi)}
 
 
@@ -66,13 +66,26 @@ def fit(sizes, hidden, guess, steps, iterations, every=50, precision="f32", r=5.
                 cur[m] = cur[m].replace(**upd)
             if log:
                 log(it, losses[-1])
-    err = {n: float(np.mean([abs(getattr(c, n) - getattr(h, n)) / max(abs(getattr(h, n)), 1e-12)
-                              for c, h in zip(cur, hidden)])) for n in NAMES}
-    err0 = {n: float(np.mean([abs(getattr(g, n) - getattr(h, n)) / max(abs(getattr(h, n)), 1e-12)
-                               for g, h in zip(guess, hidden)])) for n in NAMES}
     return {"members": M, "steps": steps, "iterations": iterations, "loss_first": losses[0],
-            "loss_last": losses[-1], "rel_param_error_start": err0, "rel_param_error_end": err,
-            "batched_adjoint_seconds": t_adj, "fitted": [{n: getattr(c, n) for n in NAMES} for c in cur]}
+            "loss_last": losses[-1], "param_error_start": param_errors(guess, hidden),
+            "param_error_end": param_errors(cur, hidden), "batched_adjoint_seconds": t_adj,
+            "fitted": [{n: getattr(c, n) for n in NAMES} for c in cur]}
+
+
+def param_errors(params, hidden):
+    """Mean relative error of delta and K, and mean absolute error (radians) of
+    theta0 modulo 2 pi / j -- the period of eps(theta - theta0)."""
+    out = {}
+    for n in ("delta", "K"):
+        out[n + "_rel"] = float(np.mean([abs(getattr(p, n) - getattr(h, n)) / abs(getattr(h, n))
+                                         for p, h in zip(params, hidden)]))
+    d = []
+    for p, h in zip(params, hidden):
+        per = 2 * np.pi / h.aniso
+        e = (p.theta0 - h.theta0) % per
+        d.append(min(e, per - e))
+    out["theta0_abs_rad"] = float(np.mean(d))
+    return out
 
 
 def looped_adjoint_seconds(sizes, params, targets, steps, every=50, precision="f32", r=5.0):
