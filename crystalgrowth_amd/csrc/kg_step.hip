@@ -3,13 +3,15 @@
 // One explicit-Euler step = the reference's pass 1 (_computeGradientLaplacian,
 // src/Kobayashi.cpp:125-175) and pass 2 (_evolution, :177-221).  The fused
 // kernel does both in one launch per step:
-//   * a 256-thread block owns a 64 x 16 output tile; phi (halo 2) and T
+//   * a 512-thread block owns a 64 x 30 output tile; phi (halo 2) and T
 //     (halo 1) are staged in LDS as rows x0-4 .. x0+67, loaded with aligned
-//     16-byte vector loads on interior tiles (mapped scalar loads on tiles
-//     touching the grid edge);
+//     16-byte vector loads whenever nx % 4 == 0 (halo segments beyond an
+//     edge wrap onto aligned segments or clamp to one edge value);
 //   * the pass-1 quantities that pass 2 reads from neighbours -- eps^2,
 //     eps eps' gx, eps eps' gy -- are computed once per cell of the tile plus
-//     a 1-cell halo (66 x 18 cells) and kept in LDS;
+//     a 1-cell halo (66 x 32 cells) and kept in LDS; atan and sin/cos run
+//     only where a branch of :154-167 fires or the kept angle is not +0
+//     (cells never fired on take eps(+0), computed once per tile);
 //   * each thread then updates a row of 4 consecutive cells with 16-byte
 //     LDS reads, one Philox block for their 4 noise samples, and one 16-byte
 //     store per field.
@@ -117,39 +119,36 @@ __device__ __forceinline__ int map_index(int x, int n)
 
 // The angle update of src/Kobayashi.cpp:154-167.  Returns true when a branch
 // fires (theta := th); false leaves the stored theta in force.  NaN
-// gradients fire nothing, as in the reference.
+// gradients fire nothing, as in the reference.  atan(gy / gx) is evaluated
+// only where a branch that uses it fires, so a wave whose cells all keep
+// their angle (flat gradients) or take the axis values skips it.
 template <typename R>
 __device__ __forceinline__ bool theta_branch(R gx, R gy, R &th)
 {
     const R E = (R)1.1920928955078125e-7;  // FLT_EPSILON in both precisions
     const R PI = Consts<R>::pi();
-    const R t = kg_atan_ratio(gy, gx);     // atan(gy / gx)
-    if (gx > E) {
-        if (gy < -E) {
-            th = (R)2.0 * PI + t;
-            return true;
-        }
-        if (gy > E) {
-            th = t;
-            return true;
-        }
-        return false;
+    const bool yneg = gy < -E, ypos = gy > E;
+    const bool xneg = gx < -E;
+    const bool with_atan = (gx > E && (yneg || ypos)) || xneg;      // :160-167
+    const bool on_axis = gx <= E && gx >= -E && (yneg || ypos);     // :154-158
+    if (with_atan) {
+        const R t = kg_atan_ratio(gy, gx);  // atan(gy / gx)
+        th = xneg ? PI + t : (yneg ? (R)2.0 * PI + t : t);
+    } else if (on_axis) {
+        th = yneg ? (R)-0.5 * PI : (R)0.5 * PI;
     }
-    if (gx < -E) {
-        th = PI + t;
-        return true;
-    }
-    if (gx <= E && gx >= -E) {
-        if (gy < -E) {
-            th = (R)-0.5 * PI;
-            return true;
-        }
-        if (gy > E) {
-            th = (R)0.5 * PI;
-            return true;
-        }
-    }
-    return false;
+    return with_atan || on_axis;
+}
+
+// eps (:170) and the factor eps eps' of the cross terms (:171 times eps)
+// for angle th
+template <typename R>
+__device__ __forceinline__ void eps_of_angle(const StepArgs<R> &a, R th, R &eps, R &f)
+{
+    R sn, cs;
+    kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
+    eps = a.eps_bar * ((R)1.0 + a.delta * cs);
+    f = eps * (a.neg_ebar_j_delta * sn);
 }
 
 // (2 (E + W + N + S) + NE + SW + NW + SE - 12 C), the order of :142-151,
@@ -180,25 +179,85 @@ struct TileSmem {
     R E2[QH * LW];
     R A[QH * LW];
     R B[QH * LW];
+    R Z[2];          // eps, eps eps' at theta = +0
+    int W[NT / 64];  // per-wave flatness bits
+};
+
+// Stage `rows` grid rows starting at y (mapped by the boundary) of field f,
+// columns x0-4 .. x0+67, into LDS rows of width LW.  With nx % 4 == 0 every
+// 4-column segment lies wholly inside or wholly outside the grid: inside it
+// is one aligned 16-byte load; outside, periodic wraps it onto another
+// aligned segment and zero-flux clamps it to one edge value.  Other widths
+// load cell by cell through the boundary map.  CHECK 1 accumulates phi's
+// flatness (all +0 / all 1) into f0/f1, CHECK 2 T's finiteness into f0.
+template <typename R, int BC, int CHECK>
+__device__ __forceinline__ void stage_rows(const R *__restrict__ f, R *__restrict__ dst, int nx,
+                                           int ny, int x0, int y, int rows, int &f0, int &f1)
+{
+    const int tid = threadIdx.x;
+    constexpr int V = LW / 4;
+    if ((nx & 3) == 0) {
+        for (int k = tid; k < rows * V; k += NT) {
+            const int r = k / V, v = k - r * V;
+            const R *row = f + (size_t)map_index<BC>(y + r, ny) * nx;
+            const int gx = x0 - 4 + 4 * v;
+            R q[4];
+            if (gx >= 0 && gx < nx) {
+                load4(row + gx, q);
+            } else if (BC == 0) {
+                load4(row + map_index<0>(gx, nx), q);
+            } else {
+                q[0] = q[1] = q[2] = q[3] = row[gx < 0 ? 0 : nx - 1];
+            }
+            store4(&dst[r * LW + 4 * v], q);
+            if (CHECK == 1)
+                flat4(q, f0, f1);
+            else if (CHECK == 2)
+                f0 &= finite4(q);
+        }
+    } else {
+        for (int k = tid; k < rows * LW; k += NT) {
+            const int r = k / LW, c = k - r * LW;
+            const R v = f[(size_t)map_index<BC>(y + r, ny) * nx + map_index<BC>(x0 - 4 + c, nx)];
+            dst[k] = v;
+            if (CHECK == 1) {
+                f0 &= is_pos_zero(v);
+                f1 &= (v == (R)1.0);
+            } else if (CHECK == 2) {
+                f0 &= isfinite(v) ? 1 : 0;
+            }
+        }
+    }
+}
+
+// eps and eps eps' at theta = +0, the angle of every cell no branch has
+// fired on yet; computed by the same code as any other angle, so cells that
+// take them get the bits the full evaluation would give.
+template <typename R>
+struct ZeroAngle {
+    R eps, f;
 };
 
 // pass-1 values of one cell from its phi neighbourhood (rows m, z, p of sP
 // around column c) and its old theta.  Returns e^2, eps eps' gx, eps eps' gy
 // and the theta the step leaves (fired: true when a branch rewrote it).
 template <typename R>
-__device__ __forceinline__ void pass1_cell(const StepArgs<R> &a, const R *m, const R *z,
-                                           const R *p, int c, R th_old, R &e2, R &A, R &B,
-                                           R &th, bool &fired)
+__device__ __forceinline__ void pass1_cell(const StepArgs<R> &a, const ZeroAngle<R> &z0,
+                                           const R *m, const R *z, const R *p, int c,
+                                           R th_old, R &e2, R &A, R &B, R &th, bool &fired)
 {
     const R gx = (z[c + 1] - z[c - 1]) * a.inv_dx;
     const R gy = (p[c] - m[c]) * a.inv_dy;
     fired = theta_branch(gx, gy, th);
     if (!fired)
         th = th_old;
-    R sn, cs;
-    kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
-    const R eps = a.eps_bar * ((R)1.0 + a.delta * cs);
-    const R f = eps * (a.neg_ebar_j_delta * sn);
+    R eps, f;
+    if (is_pos_zero(th)) {
+        eps = z0.eps;
+        f = z0.f;
+    } else {
+        eps_of_angle(a, th, eps, f);
+    }
     e2 = eps * eps;
     A = f * gx;
     B = f * gy;
@@ -218,62 +277,38 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
 
     // ---- stage phi (rows y0-2..y0+TY+1) and T (rows y0-1..y0+TY) ---------
     int all0 = 1, all1 = 1, finite = 1;
-    const bool interior = x0 >= 4 && x0 + TX + 4 <= nx && y0 >= 2 && y0 + TY + 2 <= ny &&
-                          (nx & 3) == 0;
-    constexpr int V = LW / 4;
-    if (interior) {
-        for (int k = tid; k < PH * V; k += NT) {
-            const int r = k / V, v = k - r * V;
-            R q[4];
-            load4(a.phi_in + (size_t)(y0 - 2 + r) * nx + (x0 - 4 + 4 * v), q);
-            store4(&sP[r * LW + 4 * v], q);
-            flat4(q, all0, all1);
-        }
-        for (int k = tid; k < QH * V; k += NT) {
-            const int r = k / V, v = k - r * V;
-            R q[4];
-            load4(a.T_in + (size_t)(y0 - 1 + r) * nx + (x0 - 4 + 4 * v), q);
-            store4(&sT[r * LW + 4 * v], q);
-            finite &= finite4(q);
-        }
-    } else {
-        for (int k = tid; k < PH * LW; k += NT) {
-            const int r = k / LW, c = k - r * LW;
-            const R v = a.phi_in[(size_t)map_index<BC>(y0 - 2 + r, ny) * nx +
-                                 map_index<BC>(x0 - 4 + c, nx)];
-            sP[k] = v;
-            all0 &= is_pos_zero(v);
-            all1 &= (v == (R)1.0);
-        }
-        for (int k = tid; k < QH * LW; k += NT) {
-            const int r = k / LW, c = k - r * LW;
-            const R v = a.T_in[(size_t)map_index<BC>(y0 - 1 + r, ny) * nx +
-                               map_index<BC>(x0 - 4 + c, nx)];
-            sT[k] = v;
-            finite &= isfinite(v) ? 1 : 0;
-        }
+    stage_rows<R, BC, 1>(a.phi_in, sP, nx, ny, x0, y0 - 2, PH, all0, all1);
+    stage_rows<R, BC, 2>(a.T_in, sT, nx, ny, x0, y0 - 1, QH, finite, finite);
+    // block-wide AND of the two flatness flags with one barrier: each wave
+    // posts its ballots, every thread combines the eight
+    {
+        const bool w0 = __ballot(all0 & finite) == __ballot(1);
+        const bool w1 = __ballot(all1 & finite) == __ballot(1);
+        if ((tid & 63) == 0)
+            sm.W[tid >> 6] = (w0 ? 1 : 0) | (w1 ? 2 : 0);
     }
-    const int flat0 = __syncthreads_and(all0 & finite);
-    const int flat1 = __syncthreads_and(all1 & finite);
-    const bool flat = flat0 || flat1;
+    __syncthreads();
+    int fl = 3;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w)
+        fl &= sm.W[w];
+    const bool flat0 = (fl & 1) != 0;
+    const bool flat = fl != 0;
 
     if (!flat) {
         // ---- stage the old theta of the pass-1 cells into B ---------------
-        if (interior) {
-            for (int k = tid; k < QH * V; k += NT) {
-                const int r = k / V, v = k - r * V;
-                R q[4];
-                load4(a.theta + (size_t)(y0 - 1 + r) * nx + (x0 - 4 + 4 * v), q);
-                store4(&sB[r * LW + 4 * v], q);
-            }
-        } else {
-            for (int k = tid; k < QH * LW; k += NT) {
-                const int r = k / LW, c = k - r * LW;
-                sB[k] = a.theta[(size_t)map_index<BC>(y0 - 1 + r, ny) * nx +
-                                map_index<BC>(x0 - 4 + c, nx)];
-            }
+        {
+            int unused0 = 1, unused1 = 1;
+            stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+        }
+        if (tid == 0) {
+            R e0, f0;
+            eps_of_angle(a, (R)0, e0, f0);
+            sm.Z[0] = e0;
+            sm.Z[1] = f0;
         }
         __syncthreads();
+        const ZeroAngle<R> z0{sm.Z[0], sm.Z[1]};
 
         // ---- pass 1: one 4-cell group per thread (columns x0 .. x0+63) ----
         {
@@ -296,7 +331,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 load4(pp + kc, p + 1);
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    pass1_cell(a, m, z, p, e + 1, thold[e], e2[e], A[e], B[e], th[e], fired[e]);
+                    pass1_cell(a, z0, m, z, p, e + 1, thold[e], e2[e], A[e], B[e], th[e], fired[e]);
             } else {
                 const int cy = uy < 0 ? 0 : (uy >= ny ? ny - 1 : uy);
                 const int lr = r + (cy - uy);
@@ -304,7 +339,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 for (int e = 0; e < 4; ++e) {
                     const int cx = ux + e >= nx ? nx - 1 : ux + e;
                     const int lc = kc + e + (cx - (ux + e));
-                    pass1_cell(a, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
+                    pass1_cell(a, z0, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
                                thold[e], e2[e], A[e], B[e], th[e], fired[e]);
                 }
             }
@@ -339,7 +374,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             }
             R e2, A, B, th;
             bool fired;
-            pass1_cell(a, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
+            pass1_cell(a, z0, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
                        sB[r * LW + lcol], e2, A, B, th, fired);
             sE2[r * LW + lcol] = e2;
             sA[r * LW + lcol] = A;
@@ -355,28 +390,31 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         return;
     const int kc = 4 * g + 4;  // LDS column of the first cell
     // 6-wide windows: index 0 = column i0-1, 1..4 = the cells, 5 = i0+4
-    R Pm[6], Pz[6], Pp[6], Tm[6], Tz[6], Tp[6];
+    R Tm[6], Tz[6], Tp[6];
     auto win6 = [kc](const R *rowp, R w[6]) {
         w[0] = rowp[kc - 1];
         load4(rowp + kc, w + 1);
         w[5] = rowp[kc + 4];
     };
-    win6(&sP[(row + 1) * LW], Pm);
-    win6(&sP[(row + 2) * LW], Pz);
-    win6(&sP[(row + 3) * LW], Pp);
     win6(&sT[row * LW], Tm);
     win6(&sT[(row + 1) * LW], Tz);
     win6(&sT[(row + 2) * LW], Tp);
 
     R phin[4], Tn[4];
     if (flat) {
+        // phi is the tile's constant (+0 or 1); phi' = phi
+        const R pc = flat0 ? (R)0 : (R)1;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const R lapT = lap9(Tm, Tz, Tp, e + 1) * a.inv_3dx2;
-            phin[e] = Pz[e + 1];
-            Tn[e] = Tz[e + 1] + lapT * a.dt + a.K * (phin[e] - Pz[e + 1]);
+            phin[e] = pc;
+            Tn[e] = Tz[e + 1] + lapT * a.dt + a.K * (phin[e] - pc);
         }
     } else {
+        R Pm[6], Pz[6], Pp[6];
+        win6(&sP[(row + 1) * LW], Pm);
+        win6(&sP[(row + 2) * LW], Pz);
+        win6(&sP[(row + 3) * LW], Pp);
         R E2m[4], E2z[6], E2p[4], Am[4], Ap[4], Bz[6];
         load4(&sE2[row * LW + kc], E2m);
         win6(&sE2[(row + 1) * LW], E2z);
