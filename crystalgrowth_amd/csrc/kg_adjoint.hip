@@ -636,6 +636,9 @@ int adjoint_single(kg_solver_s &h, int nsteps, int every, const void *target, kg
     job.cur_phi = static_cast<R *>(h.phi[h.cur]);
     job.cur_T = static_cast<R *>(h.T[h.cur]);
     job.cur_theta = static_cast<R *>(h.theta);
+    // the forward steps below run without tile status: it no longer describes
+    // the handle's buffers afterwards
+    h.stat_valid = false;
     const StepArgs<R> base = make_args<R>(h.params, h.nx, h.ny);
     job.fwd = [&]() -> int {
         StepArgs<R> a = base;
@@ -718,9 +721,11 @@ int adjoint_batch(kg_batch_s &b, int nsteps, int every, const void *const *targe
     for (int m = 0; m < M; ++m)
         std::memcpy(tgt.data() + b.off[m], targets[m], b.cells(m) * sizeof(R));
     std::vector<kg_grads> g;
-    if (int rc = adjoint_core<R>(job, b.adj, nsteps, every, tgt.data(), g,
-                                 d_T0 ? dT.data() : nullptr, nullptr, true))
-        return rc;
+    const int rc0 = adjoint_core<R>(job, b.adj, nsteps, every, tgt.data(), g,
+                                    d_T0 ? dT.data() : nullptr, nullptr, true);
+    b.stat_valid = false;  // the core may restore states into the batch buffers
+    if (rc0)
+        return rc0;
     for (int m = 0; m < M; ++m) {
         out[m] = g[m];
         if (d_T0 && d_T0[m])

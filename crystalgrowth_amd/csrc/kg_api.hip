@@ -75,7 +75,27 @@ kg::StepArgs<R> make_args(const kg_solver_s &h)
     a.phi_out = static_cast<R *>(h.phi[h.cur ^ 1]);
     a.T_out = static_cast<R *>(h.T[h.cur ^ 1]);
     a.theta = static_cast<R *>(h.theta);
+    if (h.stat_ok && h.kernel == KG_KERNEL_FUSED) {
+        a.stat_in = h.stat + (size_t)h.cur * h.stat_tiles;
+        a.stat_out = h.stat + (size_t)(h.cur ^ 1) * h.stat_tiles;
+        a.tile_base = 0;
+        a.tiles_x = h.tiles_x;
+        a.tiles_y = h.tiles_y;
+    }
     return a;
+}
+
+// The tile status no longer describes the buffers (fields written by
+// something other than the fused step): it is cleared before the next step.
+void invalidate_status(kg_solver_s &h) { h.stat_valid = false; }
+
+int ensure_status(kg_solver_s &h)
+{
+    if (!h.stat_ok || h.stat_valid || h.kernel != KG_KERNEL_FUSED)
+        return KG_OK;
+    KG_HIP(kg::launch_fill_u32(h.stat, 2 * (size_t)h.stat_tiles, 0u, h.stream));
+    h.stat_valid = true;
+    return KG_OK;
 }
 
 int ensure_temps(kg_solver_s &h)
@@ -93,6 +113,10 @@ int ensure_temps(kg_solver_s &h)
 template <typename R>
 int step_impl(kg_solver_s &h, int nsteps)
 {
+    if (h.kernel != KG_KERNEL_FUSED)
+        invalidate_status(h);  // the step writes the buffers without keeping the status
+    else if (int rc = ensure_status(h))
+        return rc;
     for (int s = 0; s < nsteps; ++s) {
         kg::StepArgs<R> a = make_args<R>(h);
         a.step = (uint32_t)h.steps;
@@ -130,6 +154,7 @@ int init_fields(kg_solver_s &h, int kind, int x, int y, const std::vector<int> &
                               hipMemcpyHostToDevice, h.stream));
     hipError_t e;
     h.cur = 0;
+    invalidate_status(h);
     if (h.precision == KG_F64)
         e = kg::launch_init<double>(static_cast<double *>(h.phi[0]), static_cast<double *>(h.T[0]),
                                     static_cast<double *>(h.theta), h.nx, h.ny, kind, x, y, h.disks,
@@ -155,7 +180,7 @@ void free_all(kg_solver_s *h)
     kg::free_adjoint(h->adj);
     h->adj = nullptr;
     void *bufs[] = {h->phi[0], h->phi[1], h->T[0], h->T[1], h->theta, h->temps, h->partials,
-                    h->red, h->disks};
+                    h->red, h->disks, h->stat};
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
@@ -325,6 +350,19 @@ int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out)
         free_all(h);
         return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of reduction scratch failed");
     }
+    {
+        int nt, per, nb;
+        kg::fused_geometry(nx, ny, &nt, &per, &nb);
+        h->stat_tiles = nt;
+        h->tiles_x = (nx + kg::fused_tile_x() - 1) / kg::fused_tile_x();
+        h->tiles_y = (ny + kg::fused_tile_y() - 1) / kg::fused_tile_y();
+        h->stat_ok = kg::fused_status_ok(nx, ny, p->boundary);
+        if (hipMalloc(&h->stat, 2 * (size_t)nt * sizeof(uint32_t)) != hipSuccess) {
+            h->stat = nullptr;
+            free_all(h);
+            return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the tile status failed");
+        }
+    }
     rc = kg_reset(h);
     if (rc != KG_OK) {
         free_all(h);
@@ -382,6 +420,8 @@ int kg_set_params(kg_handle h, const kg_params *p)
     if (int rc = kg_validate(h->nx, h->ny, p))
         return rc;
     h->params = *p;
+    h->stat_ok = kg::fused_status_ok(h->nx, h->ny, p->boundary);
+    invalidate_status(*h);
     return KG_OK;
 }
 
@@ -470,9 +510,11 @@ int kg_set_kernel(kg_handle h, int kernel)
 {
     if (int rc = check_handle(h))
         return rc;
-    if (kernel != KG_KERNEL_FUSED && kernel != KG_KERNEL_TWO_PASS)
+    if (kernel != KG_KERNEL_FUSED && kernel != KG_KERNEL_TWO_PASS &&
+        kernel != KG_KERNEL_FUSED_DENSE)
         return fail(KG_ERR_INVALID_ARGUMENT, "unknown kernel");
     h->kernel = kernel;
+    invalidate_status(*h);
     return KG_OK;
 }
 
@@ -506,6 +548,7 @@ int kg_set_fields(kg_handle h, const void *phi, const void *T, const void *theta
     if (bytes != h->field_bytes())
         return fail(KG_ERR_SIZE, "buffer size does not match the field size");
     KG_HIP(hipSetDevice(h->device));
+    invalidate_status(*h);
     KG_HIP(hipMemcpyAsync(h->phi[h->cur], phi, bytes, hipMemcpyHostToDevice, h->stream));
     KG_HIP(hipMemcpyAsync(h->T[h->cur], T, bytes, hipMemcpyHostToDevice, h->stream));
     if (theta)
@@ -520,6 +563,7 @@ int kg_device_fields(kg_handle h, void **phi, void **T, void **theta)
 {
     if (int rc = check_handle(h))
         return rc;
+    invalidate_status(*h);  // the caller may write through these addresses
     if (phi)
         *phi = h->phi[h->cur];
     if (T)

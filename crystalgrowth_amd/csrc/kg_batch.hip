@@ -3,7 +3,7 @@
 // All members live in one allocation per field (member offsets rounded up to
 // 256 bytes, so interior tiles keep their 16-byte vector loads).  A device
 // table holds each member's StepArgs; a tile table lists (member, x0, y0) for
-// every 64 x 16 tile of every member in member order, so the XCD-aware block
+// every 64 x 30 tile of every member in member order, so the XCD-aware block
 // order hands each XCD a contiguous run of members.  One launch steps every
 // tile of every member; KG_BATCH_GRAPH replays a captured hipGraph of an even
 // number of launches followed by a counter kernel, with the noise counter
@@ -56,7 +56,7 @@ void free_batch(kg_batch_s *b)
     kg::free_adjoint(b->adj);
     b->adj = nullptr;
     void *bufs[] = {b->phi[0], b->phi[1], b->T[0], b->T[1], b->theta, b->members,
-                    b->tiles, b->step_base, b->partials, b->red, b->disks};
+                    b->tiles, b->step_base, b->partials, b->red, b->disks, b->stat};
     for (void *p : bufs)
         if (p)
             (void)hipFree(p);
@@ -73,6 +73,8 @@ template <typename R>
 std::vector<kg::StepArgs<R>> member_args(const kg_batch_s &b)
 {
     std::vector<kg::StepArgs<R>> v(b.count);
+    const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
+    int tile_base = 0;
     for (int m = 0; m < b.count; ++m) {
         kg::StepArgs<R> a = kg::make_args<R>(b.params[m], b.nx[m], b.ny[m]);
         a.phi_in = reinterpret_cast<const R *>(b.field(b.phi[0], m));
@@ -80,13 +82,40 @@ std::vector<kg::StepArgs<R>> member_args(const kg_batch_s &b)
         a.phi_out = reinterpret_cast<R *>(b.field(b.phi[1], m));
         a.T_out = reinterpret_cast<R *>(b.field(b.T[1], m));
         a.theta = reinterpret_cast<R *>(b.field(b.theta, m));
+        // status words: set 0 describes buffer 0 (the kernel swaps the sets
+        // with the buffers on odd steps); members' tiles are numbered in
+        // the order of the tile table
+        const int tx = (b.nx[m] + TX - 1) / TX, ty = (b.ny[m] + TY - 1) / TY;
+        if (kg::fused_status_ok(b.nx[m], b.ny[m], b.params[m].boundary)) {
+            a.stat_in = b.stat;
+            a.stat_out = b.stat + b.stat_tiles;
+            a.tile_base = tile_base;
+            a.tiles_x = tx;
+            a.tiles_y = ty;
+        }
+        tile_base += tx * ty;
         v[m] = a;
     }
     return v;
 }
 
+// The tile status no longer describes the buffers: cleared before the next step.
+void invalidate_status(kg_batch_s &b) { b.stat_valid = false; }
+
+int ensure_status(kg_batch_s &b)
+{
+    if (b.stat_valid)
+        return KG_OK;
+    KG_HIP(kg::launch_fill_u32(b.stat, 2 * (size_t)b.stat_tiles, 0u, b.stream));
+    b.stat_valid = true;
+    return KG_OK;
+}
+
 int launch_steps(kg_batch_s &b, int n)
 {
+    if (n > 0)
+        if (int rc = ensure_status(b))
+            return rc;
     for (int s = 0; s < n; ++s) {
         const uint32_t add = (uint32_t)(b.steps - b.base_host);
         hipError_t e = b.precision == KG_F64
@@ -141,6 +170,8 @@ int step_impl(kg_batch_s &b, int nsteps)
             KG_HIP(hipStreamSynchronize(b.stream));
             b.base_host = b.steps;
         }
+        if (int rc = ensure_status(b))
+            return rc;
         while (nsteps >= b.graph_steps) {
             KG_HIP(hipGraphLaunch(b.exec, b.stream));
             b.steps += (uint64_t)b.graph_steps;
@@ -167,6 +198,7 @@ int init_member(kg_batch_s &b, int m, int kind, int x, int y, const std::vector<
                               hipMemcpyHostToDevice, b.stream));
     }
     const int c = b.cur();
+    invalidate_status(b);
     hipError_t e;
     if (b.precision == KG_F64)
         e = kg::launch_init<double>(reinterpret_cast<double *>(b.field(b.phi[c], m)),
@@ -264,7 +296,9 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
         }
     const size_t args_bytes = (size_t)count * (precision == KG_F64 ? sizeof(kg::StepArgs<double>)
                                                                     : sizeof(kg::StepArgs<float>));
+    b->stat_tiles = b->ntiles;
     if (hipMalloc(&b->members, args_bytes) != hipSuccess ||
+        hipMalloc(&b->stat, 2 * tiles.size() * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&b->tiles, tiles.size() * sizeof(int4)) != hipSuccess ||
         hipMalloc(&b->step_base, sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&b->partials, kg::reduce_partials_size() * sizeof(double)) != hipSuccess ||
@@ -315,6 +349,7 @@ int kg_batch_set_params(kg_batch b, int member, const kg_params *p)
     KG_HIP(hipSetDevice(b->device));
     KG_HIP(hipStreamSynchronize(b->stream));
     b->params[member] = *p;
+    invalidate_status(*b);
     hipError_t e;
     if (b->precision == KG_F64) {
         const auto v = member_args<double>(*b);
@@ -403,6 +438,7 @@ int kg_batch_set_fields(kg_batch b, int member, const void *phi, const void *T,
         return fail(KG_ERR_SIZE, "buffer size does not match the member's field size");
     KG_HIP(hipSetDevice(b->device));
     const int c = b->cur();
+    invalidate_status(*b);
     KG_HIP(hipMemcpyAsync(b->field(b->phi[c], member), phi, bytes, hipMemcpyHostToDevice,
                           b->stream));
     KG_HIP(hipMemcpyAsync(b->field(b->T[c], member), T, bytes, hipMemcpyHostToDevice, b->stream));
@@ -595,6 +631,7 @@ int kg_batch_autotune(kg_batch b, int trial_steps, int *mode, int *graph_steps)
             rc = kg_batch_step_timed(b, trial_steps, &ms);
         // restore the state before the next candidate
         b->steps = steps0;
+        invalidate_status(*b);
         const int cc = b->cur();
         if (e == hipSuccess)
             e = hipMemcpyAsync(b->phi[cc], s, fb, hipMemcpyDeviceToDevice, b->stream);

@@ -34,6 +34,11 @@ struct kg_solver_s {
     uint64_t steps = 0;
     int kernel = KG_KERNEL_FUSED;
     kg::AdjointWork *adj = nullptr;  // adjoint buffers, allocated on first use
+    // tile status of the fused kernel: 2 x tiles words (one set per buffer
+    // index); stat_valid false = unknown, cleared before the next fused step
+    uint32_t *stat = nullptr;
+    int stat_tiles = 0, tiles_x = 0, tiles_y = 0;
+    bool stat_ok = false, stat_valid = false;
 
     size_t cells() const { return (size_t)nx * (size_t)ny; }
     size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
@@ -68,6 +73,11 @@ struct kg_batch_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     kg::AdjointWork *adj = nullptr;  // batched adjoint buffers, allocated on first use
+    // tile status of the step kernel: two sets (buffer 0, buffer 1) of one
+    // word per tile of every member; stat_valid false = unknown
+    uint32_t *stat = nullptr;
+    int stat_tiles = 0;
+    bool stat_valid = false;
 
     size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
     size_t cells(int m) const { return (size_t)nx[m] * (size_t)ny[m]; }

@@ -31,6 +31,14 @@ struct StepArgs {
     R eps_bar, delta, aniso, theta0, neg_ebar_j_delta;
     R alpha_pi, gamma, teq, K, noise_a;
     uint32_t step, instance, key0, key1;
+    // Tile status words of the fused kernel (one per 64 x 30 tile, bits in
+    // kg_step.hip): stat_in describes the input buffers, stat_out the output
+    // buffers; the word of the tile at (x0, y0) sits at tile_base +
+    // (y0 / 30) * tiles_x + x0 / 64.  stat_in == nullptr: no status (every
+    // tile loads and stores all of its fields).
+    const uint32_t *stat_in;
+    uint32_t *stat_out;
+    int tile_base, tiles_x, tiles_y;
 };
 
 // Two-pass variant: pass-1 temporaries in global memory, as the reference's
@@ -69,6 +77,10 @@ hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba,
 
 // Tiles of the fused kernel and its launch geometry.
 void fused_geometry(int nx, int ny, int *ntiles, int *per_xcd, int *nblocks);
+// Tile status can be kept for an nx x ny grid with this boundary (every halo
+// of a tile lies in its 3 x 3 tile neighbourhood); fill `n` words with v.
+bool fused_status_ok(int nx, int ny, int boundary);
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 // Output tile size of the fused kernel (members of a batch are cut into it).
 int fused_tile_x();
 int fused_tile_y();

@@ -27,6 +27,8 @@
 // the same reason.
 #include <hip/hip_runtime.h>
 #include <float.h>
+
+#include <algorithm>
 #include <math.h>
 
 #include "kg_internal.h"
@@ -168,6 +170,22 @@ __device__ __forceinline__ int xcd_tile(int per_xcd)
     return (b % NXCD) * per_xcd + b / NXCD;
 }
 
+// Tile status: one word per tile and buffer, written by the block that
+// wrote the tile; a bit is set only when it holds on every cell of the tile.
+constexpr uint32_t S_PHI0 = 1u;    // phi is +0 (bit pattern)
+constexpr uint32_t S_PHI1 = 2u;    // phi is 1
+constexpr uint32_t S_T0 = 4u;      // T is +0 (bit pattern)
+constexpr uint32_t S_ACTIVE = 8u;  // the writing step took the full (non-flat) path
+
+// tile coordinate t of a neighbour, mapped like the cells (clamp or wrap)
+template <int BC>
+__device__ __forceinline__ int map_tile(int t, int n)
+{
+    if (BC == 1)
+        return t < 0 ? 0 : (t >= n ? n - 1 : t);
+    return t < 0 ? t + n : (t >= n ? t - n : t);
+}
+
 // LDS of one tile: phi (halo 2), T (halo 1) and the pass-1 values that pass 2
 // reads from neighbours, each row covering columns x0-4 .. x0+67.  The old
 // theta of non-flat tiles is staged in B: every pass-1 cell reads its own
@@ -179,9 +197,19 @@ struct TileSmem {
     R E2[QH * LW];
     R A[QH * LW];
     R B[QH * LW];
-    R Z[2];          // eps, eps eps' at theta = +0
-    int W[NT / 64];  // per-wave flatness bits
+    R Z[2];           // eps, eps eps' at theta = +0
+    int W[NT / 64];   // per-wave flatness bits
+    int W2[NT / 64];  // per-wave status bits of the outputs
 };
+
+// LDS rows of width LW all set to v
+template <typename R>
+__device__ __forceinline__ void fill_rows(R *__restrict__ dst, int rows, R v)
+{
+    const R q[4] = {v, v, v, v};
+    for (int k = threadIdx.x; k < rows * (LW / 4); k += NT)
+        store4(&dst[4 * k], q);
+}
 
 // Stage `rows` grid rows starting at y (mapped by the boundary) of field f,
 // columns x0-4 .. x0+67, into LDS rows of width LW.  With nx % 4 == 0 every
@@ -275,10 +303,78 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     const int nx = a.nx, ny = a.ny;
     const int tid = threadIdx.x;
 
+    // ---- tile status ------------------------------------------------------
+    // known: bits that hold on the input buffers over the 3 x 3 tiles around
+    // this one (so on the tile and all of its halos); has: bits that already
+    // hold on this tile of the output buffers; active: the tile took the full
+    // path last step, so theta is staged with phi and T.
+    const bool track = a.stat_in != nullptr;
+    uint32_t known = 0, has = 0;
+    bool active = false;
+    int sidx = 0;
+    if (track) {
+        const int tx = x0 / TX, ty = y0 / TY;
+        sidx = a.tile_base + ty * a.tiles_x + tx;
+        has = a.stat_out[sidx];
+        active = (a.stat_in[sidx] & S_ACTIVE) != 0;
+        uint32_t k = ~0u;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = map_tile<BC>(ty + dy, a.tiles_y);
+            for (int dx = -1; dx <= 1; ++dx)
+                k &= a.stat_in[a.tile_base + yy * a.tiles_x + map_tile<BC>(tx + dx, a.tiles_x)];
+        }
+        known = k;
+        if ((known & (S_PHI0 | S_T0)) == (S_PHI0 | S_T0) && isfinite(a.K) && isfinite(a.dt) &&
+            isfinite(a.inv_3dx2)) {
+            // phi = +0 and T = +0 on the tile and its halos: the step leaves
+            // both +0 (every term is a signed zero summing to +0); store only
+            // what the output buffers do not already hold
+            const bool w_phi = !(has & S_PHI0), w_T = !(has & S_T0);
+            if (w_phi || w_T) {
+                const int g = tid % NG, row = tid / NG;
+                const int i0 = x0 + 4 * g, j = y0 + row;
+                if (row < TY && j < ny && i0 < nx) {
+                    const R zero4[4] = {(R)0, (R)0, (R)0, (R)0};
+                    const size_t o = (size_t)j * nx + i0;
+                    if (i0 + 3 < nx && (nx & 3) == 0) {
+                        if (w_phi)
+                            store4(a.phi_out + o, zero4);
+                        if (w_T)
+                            store4(a.T_out + o, zero4);
+                    } else {
+                        for (int e = 0; e < 4 && i0 + e < nx; ++e) {
+                            if (w_phi)
+                                a.phi_out[o + e] = (R)0;
+                            if (w_T)
+                                a.T_out[o + e] = (R)0;
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // every wave has read `has` before it changes
+            if (tid == 0)
+                a.stat_out[sidx] = S_PHI0 | S_T0;
+            return;
+        }
+    }
+
     // ---- stage phi (rows y0-2..y0+TY+1) and T (rows y0-1..y0+TY) ---------
-    int all0 = 1, all1 = 1, finite = 1;
-    stage_rows<R, BC, 1>(a.phi_in, sP, nx, ny, x0, y0 - 2, PH, all0, all1);
-    stage_rows<R, BC, 2>(a.T_in, sT, nx, ny, x0, y0 - 1, QH, finite, finite);
+    // (a field known to be constant is not loaded; its LDS rows are filled
+    // only if the full path needs them)
+    const bool phi_known = (known & (S_PHI0 | S_PHI1)) != 0;
+    int all0 = (known & S_PHI0) ? 1 : 0, all1 = (known & S_PHI1) ? 1 : 0, finite = 1;
+    if (!phi_known) {
+        all0 = all1 = 1;
+        stage_rows<R, BC, 1>(a.phi_in, sP, nx, ny, x0, y0 - 2, PH, all0, all1);
+    }
+    if (known & S_T0)
+        fill_rows(sT, QH, (R)0);
+    else
+        stage_rows<R, BC, 2>(a.T_in, sT, nx, ny, x0, y0 - 1, QH, finite, finite);
+    if (active) {
+        int unused0 = 1, unused1 = 1;
+        stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+    }
     // block-wide AND of the two flatness flags with one barrier: each wave
     // posts its ballots, every thread combines the eight
     {
@@ -297,10 +393,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
 
     if (!flat) {
         // ---- stage the old theta of the pass-1 cells into B ---------------
-        {
+        if (!active) {
             int unused0 = 1, unused1 = 1;
             stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
         }
+        if (phi_known)
+            fill_rows(sP, PH, (known & S_PHI0) ? (R)0 : (R)1);
         if (tid == 0) {
             R e0, f0;
             eps_of_angle(a, (R)0, e0, f0);
@@ -386,8 +484,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     // ---- pass 2: one row of 4 cells per thread ----------------------------
     const int g = tid % NG, row = tid / NG;
     const int i0 = x0 + 4 * g, j = y0 + row;
-    if (row >= TY || j >= ny || i0 >= nx)
-        return;
+    const bool mine = row < TY && j < ny && i0 < nx;
+    uint32_t obits = S_PHI0 | S_PHI1 | S_T0;  // what holds on this thread's outputs
+    if (mine) {
     const int kc = 4 * g + 4;  // LDS column of the first cell
     // 6-wide windows: index 0 = column i0-1, 1..4 = the cells, 5 = i0+4
     R Tm[6], Tz[6], Tp[6];
@@ -468,17 +567,53 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         }
     }
 
+    // a flat tile's phi is its constant: skip the store where the output
+    // buffer already holds it
+    const bool w_phi = !(flat && (has & (flat0 ? S_PHI0 : S_PHI1)));
     const size_t o = (size_t)j * nx + i0;
     if (i0 + 3 < nx && (nx & 3) == 0) {
-        store4(a.phi_out + o, phin);
+        if (w_phi)
+            store4(a.phi_out + o, phin);
         store4(a.T_out + o, Tn);
     } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             if (i0 + e < nx) {
-                a.phi_out[o + e] = phin[e];
+                if (w_phi)
+                    a.phi_out[o + e] = phin[e];
                 a.T_out[o + e] = Tn[e];
             }
+        }
+    }
+    if (track) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (i0 + e < nx) {
+                if (!is_pos_zero(phin[e]))
+                    obits &= ~S_PHI0;
+                if (!(phin[e] == (R)1))
+                    obits &= ~S_PHI1;
+                if (!is_pos_zero(Tn[e]))
+                    obits &= ~S_T0;
+            }
+        }
+    }
+    }  // mine
+
+    // ---- status of the output buffers on this tile ------------------------
+    if (track) {
+        const uint32_t wb = (__ballot((obits & S_PHI0) != 0) == __ballot(1) ? S_PHI0 : 0u) |
+                            (__ballot((obits & S_PHI1) != 0) == __ballot(1) ? S_PHI1 : 0u) |
+                            (__ballot((obits & S_T0) != 0) == __ballot(1) ? S_T0 : 0u);
+        if ((tid & 63) == 0)
+            sm.W2[tid >> 6] = (int)wb;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t st = flat ? 0u : S_ACTIVE;
+            uint32_t acc = S_PHI0 | S_PHI1 | S_T0;
+            for (int w = 0; w < NT / 64; ++w)
+                acc &= (uint32_t)sm.W2[w];
+            a.stat_out[sidx] = st | acc;
         }
     }
 }
@@ -530,6 +665,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
         a.T_in = a.T_out;
         a.phi_out = const_cast<R *>(pi);
         a.T_out = const_cast<R *>(ti);
+        const uint32_t *si = a.stat_in;
+        a.stat_in = a.stat_out;
+        a.stat_out = const_cast<uint32_t *>(si);
     }
     if (a.boundary == 1)
         step_tile<R, 1>(a, d.y, d.z, sm);
@@ -551,6 +689,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
     const int4 d = tiles[t];
     StepArgs<R> a = members[d.x];
     a.step = step;
+    a.stat_in = nullptr;  // the buffers are not the ones the status describes
     if (a.boundary == 1)
         step_tile<R, 1>(a, d.y, d.z, sm);
     else
@@ -824,6 +963,32 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s)
 
 int fused_tile_x() { return TX; }
 int fused_tile_y() { return TY; }
+
+bool fused_status_ok(int nx, int ny, int boundary)
+{
+    // zero-flux maps halos onto the tile itself or its edge neighbours;
+    // periodic wraps a 2-cell halo onto the far tile row/column, which must
+    // then hold at least 2 cells
+    if (boundary == 1)
+        return true;
+    return nx % TX != 1 && ny % TY != 1;
+}
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s)
+{
+    if (n == 0)
+        return hipSuccess;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
 
 template <typename R>
 hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,

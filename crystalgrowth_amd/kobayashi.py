@@ -15,10 +15,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import (KG_BATCH_GRAPH, KG_BATCH_LAUNCH, KG_F32, KG_F64, KG_KERNEL_FUSED,
-                   KG_KERNEL_TWO_PASS, Grads, Params, Stats, check, lib)
+                   KG_KERNEL_FUSED_DENSE, KG_KERNEL_TWO_PASS, Grads, Params, Stats, check, lib)
 
 _PREC = {"f32": KG_F32, "float32": KG_F32, "f64": KG_F64, "float64": KG_F64}
-_KERNELS = {"fused": KG_KERNEL_FUSED, "two_pass": KG_KERNEL_TWO_PASS}
+_KERNELS = {"fused": KG_KERNEL_FUSED, "two_pass": KG_KERNEL_TWO_PASS,
+            "fused_dense": KG_KERNEL_FUSED_DENSE}
 
 
 def default_params(**overrides) -> Params:

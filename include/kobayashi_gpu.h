@@ -46,6 +46,8 @@ extern "C" {
 /* step kernels */
 #define KG_KERNEL_FUSED 0         /* one LDS-tiled launch per step (default) */
 #define KG_KERNEL_TWO_PASS 1      /* pass 1 + pass 2 through global temporaries */
+#define KG_KERNEL_FUSED_DENSE 2   /* the fused launch without tile status: every tile
+                                     loads and stores all of its fields (same results) */
 
 typedef struct kg_params {
     double dx, dy, dt;            /* ctor: _dx=_dy=0.03, _dt=timeStep (src/Kobayashi.cpp:61-63) */

@@ -1,0 +1,110 @@
+"""Tile status of the fused kernel (needs a GPU).
+
+The default fused step keeps one status word per tile and buffer (phi all +0,
+phi all 1, T all +0, tile took the full path) and uses it to skip loads and
+stores whose values are known, to skip tiles whose 3 x 3 neighbourhood is
+all zero, and to stage theta early on active tiles.  None of this may change
+a bit: every comparison here is bitwise against KG_KERNEL_FUSED_DENSE (the
+same launch without status).  Grids are cut into partial tiles on purpose,
+and the runs start from nuclei in a zero field so that all tile kinds occur
+(quiescent, phi known, T known, flat, active)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def kb():
+    import crystalgrowth_amd as kb
+    if kb.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return kb
+
+
+def run_pair(kb, nx, ny, prec, p, setup, marks):
+    """Advance a status solver and a dense solver side by side; fields at each mark."""
+    out = []
+    with kb.Solver(nx, ny, prec, p) as a, kb.Solver(nx, ny, prec, p) as b:
+        b.set_kernel("fused_dense")
+        setup(a)
+        setup(b)
+        done = 0
+        for m in marks:
+            a.step(m - done)
+            b.step(m - done)
+            done = m
+            out.append((m, a.fields(), b.fields()))
+    return out
+
+
+def assert_bitwise(pairs):
+    for m, fa, fb in pairs:
+        for name, x, y in zip(("phi", "T", "theta"), fa, fb):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (m, name)
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_status_path_matches_dense_bitwise(kb, boundary, prec):
+    nx, ny = 460, 250          # 8 x 9 tiles, the last row and column partial
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                          boundary=boundary, seed=3)
+    pairs = run_pair(kb, nx, ny, prec, p, lambda s: s.reset_seeds(3, 5.0, 8, 11),
+                     [1, 2, 3, 40, 400])
+    assert_bitwise(pairs)
+
+
+def test_periodic_grid_with_thin_edge_tiles(kb):
+    # width % 64 == 1 and height % 30 == 1: no status is kept (dense launch)
+    nx, ny = 129, 61
+    p = kb.default_params(noise_a=0.01, boundary=0, seed=5)
+    pairs = run_pair(kb, nx, ny, "f32", p, lambda s: s.reset_disk(nx - 3, ny - 2, 6.0), [1, 60])
+    assert_bitwise(pairs)
+
+
+def test_invalidation_after_external_writes(kb):
+    """set_fields, a dense step and a kernel switch in between: the status must
+    be rebuilt, not trusted."""
+    nx, ny = 300, 200
+    p = kb.default_params(noise_a=0.01, boundary=1, seed=9)
+    rng = np.random.default_rng(2)
+    with kb.Solver(nx, ny, "f32", p) as a, kb.Solver(nx, ny, "f32", p) as b:
+        b.set_kernel("fused_dense")
+        for s in (a, b):
+            s.reset_disk(60, 50, 5.0)
+            s.step(30)
+        # new fields: zero almost everywhere, a blob where the old state was zero
+        phi = np.zeros((ny, nx), np.float32)
+        phi[150:160, 200:230] = 1.0
+        T = np.zeros((ny, nx), np.float32)
+        T[20:40, 250:290] = rng.random((20, 40)).astype(np.float32) * 0.3
+        for s in (a, b):
+            s.set_fields(phi, T)
+            s.step(25)
+        assert_bitwise([(55, a.fields(), b.fields())])
+        a.set_kernel("two_pass")
+        a.step(1)
+        a.set_kernel("fused")
+        b.set_kernel("two_pass")
+        b.step(1)
+        b.set_kernel("fused_dense")
+        for s in (a, b):
+            s.step(20)
+        assert_bitwise([(76, a.fields(), b.fields())])
+
+
+def test_batch_members_match_dense_solver(kb):
+    sizes = [(200, 130), (64, 64), (130, 61), (96, 250)]
+    params = [kb.default_params(delta=0.04, aniso=6.0, noise_a=0.01, boundary=k % 2, seed=k,
+                                instance=k) for k in range(len(sizes))]
+    with kb.Batch(sizes, params, "f32") as bt:
+        bt.reset_disk(-1, -1, -1, 5.0)
+        bt.step(150)
+        got = [bt.fields(m) for m in range(len(sizes))]
+    for (nx, ny), p, g in zip(sizes, params, got):
+        with kb.Solver(nx, ny, "f32", p) as s:
+            s.set_kernel("fused_dense")
+            s.reset_disk(nx // 2, ny // 2, 5.0)
+            s.step(150)
+            assert_bitwise([(150, g, s.fields())])
