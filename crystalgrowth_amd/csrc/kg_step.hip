@@ -162,6 +162,42 @@ __device__ __forceinline__ R lap9(const R *m, const R *z, const R *p, int c)
            m[c + 1] - (R)12.0 * z[c];
 }
 
+// Pass-1 and pass-2 threads map one tile row to 16 consecutive lanes (one
+// DPP row).  A thread's left and right neighbour columns are then held by
+// the neighbouring lanes: row_shr:1 hands lane l the value of lane l-1,
+// row_shl:1 that of lane l+1; the first / last lane of each row keeps `edge`.
+// Every lane of the row must be active.
+__device__ __forceinline__ float from_left_lane(float edge, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v),
+                                                      0x111, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right_lane(float edge, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v),
+                                                      0x101, 0xf, 0xf, false));
+}
+
+// 6-wide window of the LDS row `rowp` around the 4 cells starting at column
+// kc = 4 g + 4 (g = lane within its 16-lane row): w[0] = column kc-1,
+// w[1..4] = the cells, w[5] = column kc+4.  FP32 takes the two edge values
+// from the neighbouring lanes; the row's first and last lanes use columns 3
+// and TX+4, read by every lane of the row at one address (a broadcast), so
+// no read conflicts on LDS banks.  FP64 reads them directly.
+__device__ __forceinline__ void win6(const float *rowp, int kc, float w[6])
+{
+    load4(rowp + kc, w + 1);
+    const float le = rowp[3], re = rowp[TX + 4];
+    w[0] = from_left_lane(le, w[4]);
+    w[5] = from_right_lane(re, w[1]);
+}
+__device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
+{
+    w[0] = rowp[kc - 1];
+    load4(rowp + kc, w + 1);
+    w[5] = rowp[kc + 4];
+}
+
 __device__ __forceinline__ int xcd_tile(int per_xcd)
 {
     // Block b runs on XCD b % 8; give each XCD a contiguous run of tiles so
@@ -315,15 +351,23 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     if (track) {
         const int tx = x0 / TX, ty = y0 / TY;
         sidx = a.tile_base + ty * a.tiles_x + tx;
-        has = a.stat_out[sidx];
-        active = (a.stat_in[sidx] & S_ACTIVE) != 0;
-        uint32_t k = ~0u;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = map_tile<BC>(ty + dy, a.tiles_y);
-            for (int dx = -1; dx <= 1; ++dx)
-                k &= a.stat_in[a.tile_base + yy * a.tiles_x + map_tile<BC>(tx + dx, a.tiles_x)];
+        // lanes 0..8 of every wave load the 3 x 3 neighbourhood's input
+        // words (lane 4 = this tile), lane 9 this tile's output word
+        const int lane = tid & 63;
+        uint32_t wv = 0u;
+        if (lane < 9) {
+            const int yy = map_tile<BC>(ty + lane / 3 - 1, a.tiles_y);
+            const int xx = map_tile<BC>(tx + lane % 3 - 1, a.tiles_x);
+            wv = a.stat_in[a.tile_base + yy * a.tiles_x + xx];
+        } else if (lane == 9) {
+            wv = a.stat_out[sidx];
         }
-        known = k;
+        const uint64_t nine = 0x1ffull;
+        known = ((__ballot(lane < 9 && (wv & S_PHI0)) & nine) == nine ? S_PHI0 : 0u) |
+                ((__ballot(lane < 9 && (wv & S_PHI1)) & nine) == nine ? S_PHI1 : 0u) |
+                ((__ballot(lane < 9 && (wv & S_T0)) & nine) == nine ? S_T0 : 0u);
+        has = (uint32_t)__builtin_amdgcn_readlane((int)wv, 9);
+        active = ((uint32_t)__builtin_amdgcn_readlane((int)wv, 4) & S_ACTIVE) != 0;
         if ((known & (S_PHI0 | S_T0)) == (S_PHI0 | S_T0) && isfinite(a.K) && isfinite(a.dt) &&
             isfinite(a.inv_3dx2)) {
             // phi = +0 and T = +0 on the tile and its halos: the step leaves
@@ -419,14 +463,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             // zero-flux ghost rows/columns take the values of the cell they
             // clamp to; periodic halos already hold wrapped values
             const bool shifted = BC == 1 && (uy < 0 || uy >= ny || ux + 3 >= nx);
-            if (!shifted) {
-                R m[6], z[6], p[6];
+            // every lane reads its window (the lane exchange needs all lanes)
+            R m[6], z[6], p[6];
+            {
                 const R *pm = &sP[r * LW], *pz = &sP[(r + 1) * LW], *pp = &sP[(r + 2) * LW];
-                z[0] = pz[kc - 1];
-                load4(pz + kc, z + 1);
-                z[5] = pz[kc + 4];
+                win6(pz, kc, z);
                 load4(pm + kc, m + 1);
                 load4(pp + kc, p + 1);
+            }
+            if (!shifted) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
                     pass1_cell(a, z0, m, z, p, e + 1, thold[e], e2[e], A[e], B[e], th[e], fired[e]);
@@ -482,22 +527,19 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     }
 
     // ---- pass 2: one row of 4 cells per thread ----------------------------
+    // threads past the tile's last row (or its grid edge) compute on a
+    // valid row and store nothing: every lane takes part in the lane exchange
     const int g = tid % NG, row = tid / NG;
+    const int rw = row < TY ? row : TY - 1;
     const int i0 = x0 + 4 * g, j = y0 + row;
     const bool mine = row < TY && j < ny && i0 < nx;
     uint32_t obits = S_PHI0 | S_PHI1 | S_T0;  // what holds on this thread's outputs
-    if (mine) {
     const int kc = 4 * g + 4;  // LDS column of the first cell
     // 6-wide windows: index 0 = column i0-1, 1..4 = the cells, 5 = i0+4
     R Tm[6], Tz[6], Tp[6];
-    auto win6 = [kc](const R *rowp, R w[6]) {
-        w[0] = rowp[kc - 1];
-        load4(rowp + kc, w + 1);
-        w[5] = rowp[kc + 4];
-    };
-    win6(&sT[row * LW], Tm);
-    win6(&sT[(row + 1) * LW], Tz);
-    win6(&sT[(row + 2) * LW], Tp);
+    win6(&sT[rw * LW], kc, Tm);
+    win6(&sT[(rw + 1) * LW], kc, Tz);
+    win6(&sT[(rw + 2) * LW], kc, Tp);
 
     R phin[4], Tn[4];
     if (flat) {
@@ -510,25 +552,63 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             Tn[e] = Tz[e + 1] + lapT * a.dt + a.K * (phin[e] - pc);
         }
     } else {
-        R Pm[6], Pz[6], Pp[6];
-        win6(&sP[(row + 1) * LW], Pm);
-        win6(&sP[(row + 2) * LW], Pz);
-        win6(&sP[(row + 3) * LW], Pp);
-        R E2m[4], E2z[6], E2p[4], Am[4], Ap[4], Bz[6];
-        load4(&sE2[row * LW + kc], E2m);
-        win6(&sE2[(row + 1) * LW], E2z);
-        load4(&sE2[(row + 2) * LW + kc], E2p);
-        load4(&sA[row * LW + kc], Am);
-        load4(&sA[(row + 2) * LW + kc], Ap);
-        win6(&sB[(row + 1) * LW], Bz);
+        // in phases, so that each set of LDS windows dies before the next is
+        // read (register pressure): T, phi, eps^2, then the cross terms
+        R lapT[4], Tc[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            lapT[e] = lap9(Tm, Tz, Tp, e + 1) * a.inv_3dx2;
+            Tc[e] = Tz[e + 1];
+        }
+        R phi[4], lap[4], gx[4], gy[4];
+        {
+            R Pm[6], Pz[6], Pp[6];
+            win6(&sP[(rw + 1) * LW], kc, Pm);
+            win6(&sP[(rw + 2) * LW], kc, Pz);
+            win6(&sP[(rw + 3) * LW], kc, Pp);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = e + 1;
+                phi[e] = Pz[c];
+                lap[e] = lap9(Pm, Pz, Pp, c) * a.inv_3dx2;
+                gx[e] = (Pz[c + 1] - Pz[c - 1]) * a.inv_dx;
+                gy[e] = (Pp[c] - Pm[c]) * a.inv_dy;
+            }
+        }
+        R e2c[4], t3[4];
+        {
+            R E2m[4], E2z[6], E2p[4];
+            load4(&sE2[rw * LW + kc], E2m);
+            win6(&sE2[(rw + 1) * LW], kc, E2z);
+            load4(&sE2[(rw + 2) * LW + kc], E2p);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int c = e + 1;
+                const R gEx = (E2z[c + 1] - E2z[c - 1]) * a.inv_dx;
+                const R gEy = (E2p[e] - E2m[e]) * a.inv_dy;
+                e2c[e] = E2z[c];
+                t3[e] = gEx * gx[e] + gEy * gy[e];
+            }
+        }
+        R t1[4], t2[4];
+        {
+            R Am[4], Ap[4], Bz[6];
+            load4(&sA[rw * LW + kc], Am);
+            load4(&sA[(rw + 2) * LW + kc], Ap);
+            win6(&sB[(rw + 1) * LW], kc, Bz);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                t1[e] = (Ap[e] - Am[e]) * a.inv_dy;
+                t2[e] = -(Bz[e + 2] - Bz[e]) * a.inv_dx;
+            }
+        }
 
         // the reaction matters only where phi (1 - phi) != 0 (or T is not
         // finite, which the reference would turn into NaN)
         bool need[4], any = false;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const R p = Pz[e + 1];
-            need[e] = (p * ((R)1.0 - p) != (R)0) || !isfinite(Tz[e + 1]);
+            need[e] = (phi[e] * ((R)1.0 - phi[e]) != (R)0) || !isfinite(Tc[e]);
             any |= need[e];
         }
         R chi[4] = {(R)0, (R)0, (R)0, (R)0};
@@ -542,31 +622,21 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const int c = e + 1;
-            const R phi = Pz[c], T = Tz[c];
-            const R lap = lap9(Pm, Pz, Pp, c) * a.inv_3dx2;
-            const R lapT = lap9(Tm, Tz, Tp, c) * a.inv_3dx2;
-            const R gx = (Pz[c + 1] - Pz[c - 1]) * a.inv_dx;
-            const R gy = (Pp[c] - Pm[c]) * a.inv_dy;
-            const R gEx = (E2z[c + 1] - E2z[c - 1]) * a.inv_dx;
-            const R gEy = (E2p[e] - E2m[e]) * a.inv_dy;
-            const R t1 = (Ap[e] - Am[e]) * a.inv_dy;
-            const R t2 = -(Bz[c + 1] - Bz[c - 1]) * a.inv_dx;
-            const R t3 = gEx * gx + gEy * gy;
             R react = (R)0;
             if (need[e]) {
-                const R m = a.alpha_pi * kg_atan(a.gamma * (a.teq - T));
-                R q = phi - (R)0.5 + m;
+                const R m = a.alpha_pi * kg_atan(a.gamma * (a.teq - Tc[e]));
+                R q = phi[e] - (R)0.5 + m;
                 if (a.noise_a != (R)0)
                     q = q + a.noise_a * chi[e];
-                react = phi * ((R)1.0 - phi) * q;
+                react = phi[e] * ((R)1.0 - phi[e]) * q;
             }
-            const R sum = t1 + t2 + E2z[c] * lap + t3 + react;
-            phin[e] = phi + sum * a.dt_tau;
-            Tn[e] = T + lapT * a.dt + a.K * (phin[e] - phi);
+            const R sum = t1[e] + t2[e] + e2c[e] * lap[e] + t3[e] + react;
+            phin[e] = phi[e] + sum * a.dt_tau;
+            Tn[e] = Tc[e] + lapT[e] * a.dt + a.K * (phin[e] - phi[e]);
         }
     }
 
+    if (mine) {
     // a flat tile's phi is its constant: skip the store where the output
     // buffer already holds it
     const bool w_phi = !(flat && (has & (flat0 ? S_PHI0 : S_PHI1)));
