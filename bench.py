@@ -17,8 +17,9 @@ the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the reference's
 arithmetic restated in C, one thread) on a bounded sample of the same
 workload.  Beside the headline it reports:
   batched   -- the 512-member ensemble (sides 128..1024, per-member
-               parameters), a shortened run: one batched launch per step
-               (autotuned launch mode) vs looping the plain solver per member
+               parameters) over its 5000-step horizon from disk nuclei: one
+               batched launch per step (autotuned launch mode) vs looping the
+               plain solver per member (one handle and stream per member)
   adjoint   -- loss + gradients of 1000 steps on 512^2 (checkpoint every 50)
                vs the plain forward of the same steps
 --calibrate-copy runs only a device copy of known size (for scaling the
@@ -52,14 +53,17 @@ def parse():
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--kernel", default="fused", choices=["fused", "two_pass"])
+    ap.add_argument("--kernel", default="fused", choices=["fused", "fused_dense", "two_pass"])
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="steps of the CPU oracle sample (0 skips the CPU baseline)")
     ap.add_argument("--no-fp64-check", action="store_true")
-    ap.add_argument("--batch-steps", type=int, default=200,
+    ap.add_argument("--batch-steps", type=int, default=5000,
                     help="steps of the batched-vs-looped comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--sync-every", type=int, default=0,
+                    help="synchronise after every N timed steps (for counter-collection "
+                         "profiler runs; 0 = queue all steps)")
     return ap.parse_args()
 
 
@@ -141,23 +145,27 @@ def calibrate_copy():
 
 
 def batched_vs_looped(kb, nsteps: int):
+    """The ensemble from its initial state (disk r = 5 at every member's
+    centre) for nsteps steps: one batched launch per step vs each member's
+    own plain handle stepped in turn (every handle has its own stream, so
+    members overlap on the GPU).  Both timed from the first step to the
+    last, states resident on the device."""
     from crystalgrowth_amd import configs
     sizes, params = configs.c2_members()
     cells = sum(a * b for a, b in sizes)
     with kb.Batch(sizes, params, "f32") as b:
         b.reset_disk(-1, -1, -1, 5.0)
-        mode, g = b.autotune(32)
+        mode, g = b.autotune(32)          # restores the state it ran from
         b.reset_disk(-1, -1, -1, 5.0)
-        b.step(8)
         b.synchronize()
         t0 = time.perf_counter()
         b.step(nsteps)
         b.synchronize()
         tb = time.perf_counter() - t0
+        sb = [b.stats(m)["solid_fraction"] for m in (0, len(sizes) // 2, len(sizes) - 1)]
     solvers = [kb.Solver(nx, ny, "f32", p) for (nx, ny), p in zip(sizes, params)]
     for (nx, ny), s in zip(sizes, solvers):
         s.reset_disk(nx // 2, ny // 2, 5.0)
-        s.step(8)
     for s in solvers:
         s.synchronize()
     t0 = time.perf_counter()
@@ -166,13 +174,17 @@ def batched_vs_looped(kb, nsteps: int):
     for s in solvers:
         s.synchronize()
     tl = time.perf_counter() - t0
+    sl = [solvers[m].stats()["solid_fraction"] for m in (0, len(sizes) // 2, len(sizes) - 1)]
     for s in solvers:
         s.close()
     return {"members": len(sizes), "cells": cells, "steps": nsteps,
             "batched_Gcell_updates_per_s": round(cells * nsteps / tb / 1e9, 3),
             "looped_Gcell_updates_per_s": round(cells * nsteps / tl / 1e9, 3),
             "speedup_vs_looped": round(tl / tb, 3), "launch_mode": [mode, g],
-            "note": "per-member parity with the plain solver is bitwise (tests/test_gpu_batch.py)"}
+            "batched_s": round(tb, 3), "looped_s": round(tl, 3),
+            "same_state": sb == sl,
+            "note": "members are bitwise equal to plain handles (tests/test_gpu_batch.py); "
+                    "same_state compares three members' solid fractions at the end"}
 
 
 def adjoint_vs_forward(kb):
@@ -231,7 +243,14 @@ def main():
         torch_sync()
         s.synchronize()
         t0 = time.perf_counter()
-        ev_ms = s.step_timed(args.steps)  # HIP events on the solver's stream
+        if args.sync_every > 0:
+            ev_ms, done = 0.0, 0
+            while done < args.steps:
+                k = min(args.sync_every, args.steps - done)
+                ev_ms += s.step_timed(k)
+                done += k
+        else:
+            ev_ms = s.step_timed(args.steps)  # HIP events on the solver's stream
         s.synchronize()
         torch_sync()
         wall = time.perf_counter() - t0
@@ -262,7 +281,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(n),
-                     "kernel": "k_step_fused<float>" if args.kernel == "fused" else "k_pass1+k_pass2",
+                     "kernel": "k_pass1+k_pass2" if args.kernel == "two_pass" else "k_step_fused<float>",
                      "avg_launch_us": round(ms_per_step * 1e3, 3),
                      "alg_bytes_per_cell_update": ALG_BYTES_FP32},
         "final_state": {"solid_fraction": st["solid_fraction"], "sum_phi": st["sum_phi"],
