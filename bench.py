@@ -15,7 +15,8 @@ region; `roofline` prices the fused step kernel's average launch (HIP events
 on the solver's own stream) at 16 algorithmic bytes per cell-update against
 the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the reference's
 arithmetic restated in C, one thread) on a bounded sample of the same
-workload.  Beside the headline it reports:
+workload (`cpu_all_cores`: the same oracle on OpenMP threads, labelled
+apart).  Beside the headline it reports:
   batched   -- the 512-member ensemble (sides 128..1024, per-member
                parameters) over its 5000-step horizon from disk nuclei: one
                batched launch per step (autotuned launch mode) vs looping the
@@ -112,6 +113,24 @@ def cpu_baseline(size: int, nsteps: int):
             "sample": f"CPU oracle (reference float arithmetic restated in C, gcc -O2, 1 thread) "
                       f"on the {size}x{size} workload, steps 0-{nsteps - 1} from the seeded "
                       f"initial state, {dt:.1f} s on '{cpu_model()}' ({os.cpu_count()} cores)"}
+
+
+def cpu_all_cores(size: int, nsteps: int):
+    """The same oracle with its row loops on OpenMP threads (bitwise the
+    single-thread results; tests/test_oracle_parallel.py), labelled apart
+    from cpu_baseline: the reference itself runs on one thread."""
+    from oracle import oracle as O
+    p = O.default_params(**WORKLOAD)
+    s, _ = O.init_seeds(size, size, SEEDS["count"], SEEDS["r"], SEEDS["margin"], SEEDS["seed"],
+                        dtype=np.float32)
+    threads = O.threads_parallel()
+    t0 = time.perf_counter()
+    O.run_parallel(s, p, nsteps)
+    dt = time.perf_counter() - t0
+    return {"value": size * size * nsteps / dt / 1e9, "unit": "Gcell-updates/s",
+            "cores": threads, "kind": "port",
+            "sample": f"CPU oracle built with OpenMP over rows, {threads} threads, on the "
+                      f"{size}x{size} workload, steps 0-{nsteps - 1}, {dt:.1f} s"}
 
 
 def fp64_error(kb, size: int, nsteps: int = 100):
@@ -294,6 +313,8 @@ def main():
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_vs_forward(kb)
     rec["cpu_baseline"] = cpu_baseline(n, args.cpu_steps) if args.cpu_steps > 0 else None
+    if args.cpu_steps > 0:
+        rec["cpu_all_cores"] = cpu_all_cores(n, 6 * args.cpu_steps)
     print(json.dumps(rec), flush=True)
 
 

@@ -5,6 +5,9 @@
  *
  * Build (oracle/Makefile): gcc -O2 -ffp-contract=off -fPIC -shared, glibc libm.
  */
+#ifdef KGO_OPENMP
+#include <omp.h>
+#endif
 #include "kobayashi_oracle.h"
 
 #include <float.h>
@@ -15,6 +18,15 @@
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
 #endif
+
+int kgo_threads(void)
+{
+#ifdef KGO_OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 void kgo_default_params(kgo_params *p)
 {

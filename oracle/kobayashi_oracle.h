@@ -46,6 +46,10 @@ typedef struct kgo_params {
  * eps_bar=0.01, mu=1, K=1.6, delta=0.05, j=6, alpha=0.9, gamma=10, Teq=1. */
 void kgo_default_params(kgo_params *p);
 
+/* Threads the step loops use: 1, or the OpenMP team size of the build with
+ * -DKGO_OPENMP (libkg_oracle_omp.so, bitwise the same results). */
+int kgo_threads(void);
+
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), one block. */
 void kgo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

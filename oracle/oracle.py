@@ -54,6 +54,39 @@ def build() -> None:
 
 
 _lib = None
+_lib_omp = None
+OMP_PATH = os.path.join(HERE, "build", "libkg_oracle_omp.so")
+
+
+def lib_parallel():
+    """The same oracle with its row loops on OpenMP threads (bitwise the
+    serial results); used for the all-cores CPU timing."""
+    global _lib_omp
+    if _lib_omp is None:
+        if not os.path.exists(OMP_PATH):
+            build()
+        L = ctypes.CDLL(OMP_PATH)
+        P = ctypes.POINTER(Params)
+        for suf, dt in (("f32", np.float32), ("f64", np.float64)):
+            ap = np.ctypeslib.ndpointer(dt, flags="C_CONTIGUOUS")
+            getattr(L, f"kgo_run_{suf}").argtypes = [
+                ctypes.c_int, ctypes.c_int, P, ap, ap, ap, ctypes.c_uint32, ctypes.c_int]
+        L.kgo_threads.restype = ctypes.c_int
+        _lib_omp = L
+    return _lib_omp
+
+
+def run_parallel(state: "State", p: Params, nsteps: int, step0: int = 0) -> "State":
+    s = state.copy()
+    ny, nx = s.shape
+    suf = _dtype_suffix(s.phi.dtype)
+    getattr(lib_parallel(), f"kgo_run_{suf}")(nx, ny, ctypes.byref(p), s.phi, s.T, s.theta,
+                                              step0, nsteps)
+    return s
+
+
+def threads_parallel() -> int:
+    return int(lib_parallel().kgo_threads())
 
 
 def lib():
