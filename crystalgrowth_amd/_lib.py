@@ -120,6 +120,7 @@ SIGNATURES = {
     "kg_get_stream": [H, ctypes.POINTER(VP)],
     "kg_reduce": [H, ctypes.POINTER(Stats)],
     "kg_colormap": [H, VP, SZ, I],
+    "kg_front_profile": [H, ctypes.c_double, ctypes.c_double, I, ctypes.c_double, VP],
     "kg_adjoint": [H, I, I, VP, SZ, ctypes.POINTER(Grads), VP, VP],
     "kg_batch_create": [I, PI, PI, I, P, ctypes.POINTER(H)],
     "kg_batch_destroy": [H],
@@ -134,6 +135,7 @@ SIGNATURES = {
     "kg_batch_get_temperature": [H, I, VP, SZ],
     "kg_batch_get_angle": [H, I, VP, SZ],
     "kg_batch_reduce": [H, I, ctypes.POINTER(Stats)],
+    "kg_batch_front_profile": [H, I, ctypes.c_double, ctypes.c_double, I, ctypes.c_double, VP],
     "kg_batch_set_mode": [H, I, I],
     "kg_batch_get_mode": [H, PI, PI],
     "kg_batch_autotune": [H, I, PI, PI],

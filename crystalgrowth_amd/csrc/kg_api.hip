@@ -645,4 +645,32 @@ int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order)
     return KG_OK;
 }
 
+int kg_front_profile(kg_handle h, double cx, double cy, int n, double level, double *radii)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!radii || n < 1 || n > (1 << 24))
+        return fail(KG_ERR_INVALID_ARGUMENT, "need 1 <= n <= 2^24 and a radii buffer");
+    if (!std::isfinite(cx) || !std::isfinite(cy) || !std::isfinite(level))
+        return fail(KG_ERR_INVALID_ARGUMENT, "non-finite centre or level");
+    KG_HIP(hipSetDevice(h->device));
+    double *d = nullptr;
+    KG_HIP(hipMalloc(&d, (size_t)n * sizeof(double)));
+    hipError_t e = h->precision == KG_F64
+                       ? kg::launch_front_profile<double>(static_cast<const double *>(h->phi[h->cur]),
+                                                          h->nx, h->ny, cx, cy, n, level, d,
+                                                          h->stream)
+                       : kg::launch_front_profile<float>(static_cast<const float *>(h->phi[h->cur]),
+                                                         h->nx, h->ny, cx, cy, n, level, d,
+                                                         h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(radii, d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return hip_fail(e, "front profile");
+    return KG_OK;
+}
+
 }  // extern "C"

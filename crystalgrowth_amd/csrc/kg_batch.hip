@@ -472,6 +472,37 @@ int kg_batch_get_angle(kg_batch b, int member, void *dst, size_t bytes)
     return copy_member_out(b, member, b->theta, dst, bytes);
 }
 
+int kg_batch_front_profile(kg_batch b, int member, double cx, double cy, int n, double level,
+                           double *radii)
+{
+    if (int rc = check_batch(b, member, false))
+        return rc;
+    if (!radii || n < 1 || n > (1 << 24))
+        return fail(KG_ERR_INVALID_ARGUMENT, "need 1 <= n <= 2^24 and a radii buffer");
+    if (!std::isfinite(cx) || !std::isfinite(cy) || !std::isfinite(level))
+        return fail(KG_ERR_INVALID_ARGUMENT, "non-finite centre or level");
+    KG_HIP(hipSetDevice(b->device));
+    const int c = b->cur();
+    double *d = nullptr;
+    KG_HIP(hipMalloc(&d, (size_t)n * sizeof(double)));
+    hipError_t e =
+        b->precision == KG_F64
+            ? kg::launch_front_profile<double>(
+                  reinterpret_cast<const double *>(b->field(b->phi[c], member)), b->nx[member],
+                  b->ny[member], cx, cy, n, level, d, b->stream)
+            : kg::launch_front_profile<float>(
+                  reinterpret_cast<const float *>(b->field(b->phi[c], member)), b->nx[member],
+                  b->ny[member], cx, cy, n, level, d, b->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(radii, d, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess)
+        return hip_fail(e, "batch front profile");
+    return KG_OK;
+}
+
 int kg_batch_reduce(kg_batch b, int member, kg_stats *out)
 {
     if (int rc = check_batch(b, member, false))

@@ -75,6 +75,12 @@ hipError_t launch_reduce(const R *phi, const R *T, size_t n, double K, double *p
 template <typename R>
 hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba, hipStream_t s);
 
+// Front distances along n rays from (cx, cy) to phi = level (FP64 results,
+// `radii` device memory of n doubles).
+template <typename R>
+hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double cy, int n,
+                                double level, double *radii, hipStream_t s);
+
 // Tiles of the fused kernel and its launch geometry.
 void fused_geometry(int nx, int ny, int *ntiles, int *per_xcd, int *nblocks);
 // Tile status can be kept for an nx x ny grid with this boundary (every halo

@@ -1131,6 +1131,62 @@ hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba,
     return hipGetLastError();
 }
 
+// ---- front profile (morphology) ---------------------------------------------
+// Distance from (cx, cy) to the first point where phi drops below `level`
+// along n equally spaced rays: march in steps of 1/4 cell, sample phi
+// bilinearly (points outside the grid read 0), interpolate linearly between
+// the last two samples.  FP64 throughout; one thread per ray.
+template <typename R>
+__global__ void k_front_profile(const R *__restrict__ phi, int nx, int ny, double cx, double cy,
+                                int n, double level, double *__restrict__ radii)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    const double ang = 2.0 * 3.14159265358979323846 * (double)k / (double)n;
+    const double c = cos(ang), sn = sin(ang);
+    const double rmax = sqrt((double)nx * nx + (double)ny * ny);
+    const long nsteps = (long)ceil(rmax / 0.25);
+    double prev = 0.0;
+    double out = (double)(nsteps - 1) * 0.25;  // no crossing: the last step
+    for (long i = 0; i < nsteps; ++i) {
+        const double st = 0.25 * (double)i;
+        const double x = cx + c * st, y = cy + sn * st;
+        double v = 0.0;
+        if (x >= 0.0 && x <= (double)(nx - 1) && y >= 0.0 && y <= (double)(ny - 1)) {
+            int x0 = (int)floor(x), y0 = (int)floor(y);
+            x0 = x0 < 0 ? 0 : (x0 > nx - 2 ? nx - 2 : x0);
+            y0 = y0 < 0 ? 0 : (y0 > ny - 2 ? ny - 2 : y0);
+            double fx = x - x0, fy = y - y0;
+            fx = fx < 0.0 ? 0.0 : (fx > 1.0 ? 1.0 : fx);
+            fy = fy < 0.0 ? 0.0 : (fy > 1.0 ? 1.0 : fy);
+            const size_t o = (size_t)y0 * nx + x0;
+            v = (double)phi[o] * (1.0 - fx) * (1.0 - fy) + (double)phi[o + 1] * fx * (1.0 - fy) +
+                (double)phi[o + nx] * (1.0 - fx) * fy + (double)phi[o + nx + 1] * fx * fy;
+        }
+        if (v < level) {
+            if (i == 0) {
+                out = 0.0;
+            } else {
+                const double t = prev != v ? (prev - level) / (prev - v) : 0.0;
+                out = 0.25 * (double)(i - 1) + t * 0.25;
+            }
+            break;
+        }
+        prev = v;
+    }
+    radii[k] = out;
+}
+
+template <typename R>
+hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double cy, int n,
+                                double level, double *radii, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_front_profile<R>), dim3((n + 63) / 64), dim3(64), 0, s, phi, nx, ny, cx,
+                       cy, n, level, radii);
+    return hipGetLastError();
+}
+
 #define KG_INSTANTIATE(R)                                                                        \
     template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t);                  \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
@@ -1143,7 +1199,9 @@ hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba,
                                        double, hipStream_t);                                     \
     template hipError_t launch_reduce<R>(const R *, const R *, size_t, double, double *,         \
                                          ReduceOut *, hipStream_t);                              \
-    template hipError_t launch_colormap<R>(const R *, int, int, int, float *, hipStream_t);
+    template hipError_t launch_colormap<R>(const R *, int, int, int, float *, hipStream_t);       \
+    template hipError_t launch_front_profile<R>(const R *, int, int, double, double, int, double,  \
+                                                double *, hipStream_t);
 KG_INSTANTIATE(float)
 KG_INSTANTIATE(double)
 

@@ -180,6 +180,14 @@ class Solver:
         check(lib().kg_reduce(self._h, ctypes.byref(s)))
         return s.as_dict()
 
+    def front_profile(self, cx: float, cy: float, n: int = 720, level: float = 0.5) -> np.ndarray:
+        """Distance from (cx, cy) to the phi = level front along n rays, on the
+        device (the algorithm of morphology.radius_profile)."""
+        out = np.empty(int(n), np.float64)
+        check(lib().kg_front_profile(self._h, float(cx), float(cy), int(n), float(level),
+                                     out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
     def colormap(self, order: int = 0) -> np.ndarray:
         out = np.empty((self.nx * self.ny, 4), np.float32)
         check(lib().kg_colormap(self._h, out.ctypes.data_as(ctypes.c_void_p), out.nbytes, order))
@@ -312,6 +320,13 @@ class Batch:
         s = Stats()
         check(lib().kg_batch_reduce(self._h, member, ctypes.byref(s)))
         return s.as_dict()
+
+    def front_profile(self, member: int, cx: float, cy: float, n: int = 720,
+                      level: float = 0.5) -> np.ndarray:
+        out = np.empty(int(n), np.float64)
+        check(lib().kg_batch_front_profile(self._h, int(member), float(cx), float(cy), int(n),
+                                           float(level), out.ctypes.data_as(ctypes.c_void_p)))
+        return out
 
     def set_params(self, member: int, p: Params) -> None:
         check(lib().kg_batch_set_params(self._h, member, ctypes.byref(p)))

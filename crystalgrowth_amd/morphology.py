@@ -12,6 +12,8 @@ they agree on shape:
                    `prominence` x (max - min radius) -- the number of primary arms
                    (side branches of grown dendrites stand out far less)
   tip_velocity     growth rate of tip_radius between two snapshots
+device_summary computes the same quantities on the GPU (kg_reduce and
+kg_front_profile, the latter the radius_profile algorithm in FP64).
 """
 from __future__ import annotations
 
@@ -66,7 +68,14 @@ def arm_count(phi: np.ndarray, cx: float, cy: float, n: int = 720,
     """Number of circular peaks of the front radius whose height above the
     neighbouring minima exceeds `prominence` x (max - min radius); a profile
     whose relief (max - min) is below `min_relief` x mean radius has no arms."""
-    r = radius_profile(phi, cx, cy, n)
+    return arms_of_profile(radius_profile(phi, cx, cy, n), prominence, min_relief)
+
+
+def arms_of_profile(r: np.ndarray, prominence: float = 0.3, min_relief: float = 0.15) -> int:
+    """arm_count of an already computed radius profile (e.g. the device's
+    Solver.front_profile)."""
+    r = np.asarray(r, dtype=np.float64)
+    n = len(r)
     if r.max() <= 0 or r.max() - r.min() < min_relief * r.mean():
         return 0
     thr = prominence * (r.max() - r.min())
@@ -104,3 +113,11 @@ def tip_velocity(phi_a: np.ndarray, phi_b: np.ndarray, cx: float, cy: float,
 def summary(phi: np.ndarray, cx: float, cy: float) -> dict:
     return {"solid_fraction": solid_fraction(phi), "tip_radius": tip_radius(phi, cx, cy),
             "arm_count": arm_count(phi, cx, cy)}
+
+
+def device_summary(solver, cx: float, cy: float, n: int = 720) -> dict:
+    """summary() computed on the GPU: FP64 reductions (kg_reduce) and the
+    front profile (kg_front_profile); no field crosses PCIe."""
+    r = solver.front_profile(cx, cy, n)
+    return {"solid_fraction": solver.stats()["solid_fraction"], "tip_radius": float(r.max()),
+            "arm_count": arms_of_profile(r)}

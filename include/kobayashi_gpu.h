@@ -155,6 +155,13 @@ int kg_reduce(kg_handle h, kg_stats *out);
  * fastest; order 1: the reference's constant-buffer order, entry i showing
  * _phi[_INDEX(i / n, i % n)] (square grids only). */
 int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order);
+/* Morphology of the phi = level front (no counterpart in the reference, which
+ * only draws the field; the long-horizon comparisons use it): radii[k] = the
+ * distance in cells from (cx, cy) to the first point where phi < level along
+ * the ray at angle 2 pi k / n, marched in steps of 1/4 cell with bilinear
+ * sampling (outside the grid reads 0), linearly interpolated; FP64.  The tip
+ * radius is the largest entry; arm counts come from the profile's peaks. */
+int kg_front_profile(kg_handle h, double cx, double cy, int n, double level, double *radii);
 
 /* ---- adjoint (reverse mode) ------------------------------------------------------------
  * No counterpart in the reference (it has no gradients); needed to fit model
@@ -205,6 +212,9 @@ int kg_batch_get_phase(kg_batch b, int member, void *dst, size_t bytes);
 int kg_batch_get_temperature(kg_batch b, int member, void *dst, size_t bytes);
 int kg_batch_get_angle(kg_batch b, int member, void *dst, size_t bytes);
 int kg_batch_reduce(kg_batch b, int member, kg_stats *out);
+/* kg_front_profile of one member */
+int kg_batch_front_profile(kg_batch b, int member, double cx, double cy, int n, double level,
+                           double *radii);
 /* mode KG_BATCH_LAUNCH or KG_BATCH_GRAPH (graph_steps even, >= 2) */
 int kg_batch_set_mode(kg_batch b, int mode, int graph_steps);
 int kg_batch_get_mode(kg_batch b, int *mode, int *graph_steps);

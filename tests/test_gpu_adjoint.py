@@ -166,7 +166,7 @@ def test_torch_autograd_function(kb):
     with kb.Solver(nx, ny, "f64", base) as s:
         s.set_fields(phi0, T0)
         g, dT, _ = s.adjoint(n, target, 10, want_T0=True)
-    assert float(L) == g["loss"]
+    assert L.item() == g["loss"]
     assert p.grad.tolist() == [g["d_delta"], g["d_K"]]
     assert np.array_equal(t0.grad.numpy(), dT)
 
