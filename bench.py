@@ -23,6 +23,8 @@ apart).  Beside the headline it reports:
                plain solver per member (one handle and stream per member)
   adjoint   -- loss + gradients of 1000 steps on 512^2 (checkpoint every 50)
                vs the plain forward of the same steps
+  reference_config -- the reference's configuration (300^2 FP64, 2000 steps)
+               on the GPU and on the CPU oracle, compared cell by cell
 --calibrate-copy runs only a device copy of known size (for scaling the
 rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes).
 """
@@ -131,6 +133,44 @@ def cpu_all_cores(size: int, nsteps: int):
             "cores": threads, "kind": "port",
             "sample": f"CPU oracle built with OpenMP over rows, {threads} threads, on the "
                       f"{size}x{size} workload, steps 0-{nsteps - 1}, {dt:.1f} s"}
+
+
+def reference_config(kb, cpu_steps: int):
+    """The reference's own configuration (300^2 FP64, disk r = 5, 2000 steps,
+    crystalgrowth_amd/configs.py c0): the GPU run, the CPU oracle's full run
+    (OpenMP build, bitwise the serial oracle) compared per cell, and the
+    oracle's single-thread rate on its first cpu_steps steps."""
+    from crystalgrowth_amd import configs
+    from oracle import oracle as O
+    c = configs.C0
+    n, steps = c["n"], c["steps"]
+    p = configs.c0_params()
+    with kb.Solver(n, n, "f64", p) as s:
+        s.reset_disk(n // 2, n // 2, c["disk_r"])
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.synchronize()
+        tg = time.perf_counter() - t0
+        gphi, gT = s.phase(), s.temperature()
+    op = O.default_params(**{k: getattr(p, k) for k in ("dx", "dy", "dt", "tau", "eps_bar", "K",
+                                                         "delta", "aniso", "alpha", "gamma",
+                                                         "teq", "theta0", "noise_a",
+                                                         "boundary", "seed")})
+    st = O.init_disk(n, n, n // 2, n // 2, c["disk_r"], np.float64)
+    t0 = time.perf_counter()
+    ref = O.run_parallel(st, op, steps)
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.run(st, op, cpu_steps)
+    t1 = time.perf_counter() - t0
+    return {"grid": [n, n], "steps": steps, "precision": "f64",
+            "gpu_s": round(tg, 4), "gpu_Gcell_updates_per_s": round(n * n * steps / tg / 1e9, 4),
+            "cpu_oracle_all_cores_s": round(tp, 3), "cpu_threads": O.threads_parallel(),
+            "cpu_oracle_1_thread_Gcell_updates_per_s": round(n * n * cpu_steps / t1 / 1e9, 5),
+            "max_abs_diff_phi": float(np.max(np.abs(gphi - ref.phi))),
+            "max_abs_diff_T": float(np.max(np.abs(gT - ref.T))),
+            "solid_fraction": float((gphi > 0.5).mean())}
 
 
 def fp64_error(kb, size: int, nsteps: int = 100):
@@ -315,6 +355,7 @@ def main():
     rec["cpu_baseline"] = cpu_baseline(n, args.cpu_steps) if args.cpu_steps > 0 else None
     if args.cpu_steps > 0:
         rec["cpu_all_cores"] = cpu_all_cores(n, 6 * args.cpu_steps)
+        rec["reference_config"] = reference_config(kb, 20 * args.cpu_steps)
     print(json.dumps(rec), flush=True)
 
 
