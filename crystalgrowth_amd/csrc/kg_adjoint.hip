@@ -39,7 +39,10 @@ namespace kg {
 namespace {
 
 constexpr int ADJ_THREADS = 256;
-constexpr int ADJ_CHUNK = 1024;  // cells per block (4 per thread)
+#ifndef KG_ADJ_CHUNK
+#define KG_ADJ_CHUNK 256
+#endif
+constexpr int ADJ_CHUNK = KG_ADJ_CHUNK;  // cells per block
 enum { P_TAU, P_EPS_BAR, P_K, P_DELTA, P_THETA0, P_ALPHA, P_GAMMA, P_TEQ, P_NOISE, NPAR };
 
 template <typename R>
