@@ -108,3 +108,15 @@ def test_batch_members_match_dense_solver(kb):
             s.reset_disk(nx // 2, ny // 2, 5.0)
             s.step(150)
             assert_bitwise([(150, g, s.fields())])
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_scalar_staging_and_tiny_grids(kb, boundary):
+    """Widths that are not a multiple of 4 stage cell by cell; 3 x 3 and
+    5 x 4 grids are one tile whose neighbours are itself."""
+    p = kb.default_params(delta=0.04, aniso=6.0, noise_a=0.01, boundary=boundary, seed=8)
+    for nx, ny, setup in ((301, 77, lambda s: s.reset_seeds(2, 4.0, 6, 5)),
+                          (3, 3, lambda s: s.reset()),
+                          (5, 4, lambda s: s.reset())):
+        pairs = run_pair(kb, nx, ny, "f32", p, setup, [1, 3, 60])
+        assert_bitwise(pairs)
