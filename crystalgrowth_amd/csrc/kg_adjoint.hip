@@ -642,7 +642,8 @@ int adjoint_single(kg_solver_s &h, int nsteps, int every, const void *target, kg
     // the forward steps below run without tile status: it no longer describes
     // the handle's buffers afterwards
     h.stat_valid = false;
-    const StepArgs<R> base = make_args<R>(h.params, h.nx, h.ny);
+    StepArgs<R> base = make_args<R>(h.params, h.nx, h.ny);
+    base.wide_angles = wide_angles(h.params, h.theta_given);
     job.fwd = [&]() -> int {
         StepArgs<R> a = base;
         a.phi_in = static_cast<const R *>(h.phi[h.cur]);
@@ -707,6 +708,7 @@ int adjoint_batch(kg_batch_s &b, int nsteps, int every, const void *const *targe
             std::vector<StepArgs<R>> v(M);
             for (int m = 0; m < M; ++m) {
                 StepArgs<R> a = make_args<R>(b.params[m], b.nx[m], b.ny[m]);
+                a.wide_angles = wide_angles(b.params[m], b.theta_given[m]);
                 a.phi_in = in[0] + b.off[m];
                 a.T_in = in[1] + b.off[m];
                 a.phi_out = o[0] + b.off[m];

@@ -70,6 +70,7 @@ template <typename R>
 kg::StepArgs<R> make_args(const kg_solver_s &h)
 {
     kg::StepArgs<R> a = kg::make_args<R>(h.params, h.nx, h.ny);
+    a.wide_angles = kg::wide_angles(h.params, h.theta_given);
     a.phi_in = static_cast<const R *>(h.phi[h.cur]);
     a.T_in = static_cast<const R *>(h.T[h.cur]);
     a.phi_out = static_cast<R *>(h.phi[h.cur ^ 1]);
@@ -154,6 +155,7 @@ int init_fields(kg_solver_s &h, int kind, int x, int y, const std::vector<int> &
                               hipMemcpyHostToDevice, h.stream));
     hipError_t e;
     h.cur = 0;
+    h.theta_given = 0.0;
     invalidate_status(h);
     if (h.precision == KG_F64)
         e = kg::launch_init<double>(static_cast<double *>(h.phi[0]), static_cast<double *>(h.T[0]),
@@ -551,6 +553,7 @@ int kg_set_fields(kg_handle h, const void *phi, const void *T, const void *theta
     invalidate_status(*h);
     KG_HIP(hipMemcpyAsync(h->phi[h->cur], phi, bytes, hipMemcpyHostToDevice, h->stream));
     KG_HIP(hipMemcpyAsync(h->T[h->cur], T, bytes, hipMemcpyHostToDevice, h->stream));
+    h->theta_given = theta ? kg::max_abs(theta, h->cells(), h->precision) : 0.0;
     if (theta)
         KG_HIP(hipMemcpyAsync(h->theta, theta, bytes, hipMemcpyHostToDevice, h->stream));
     else
@@ -564,6 +567,7 @@ int kg_device_fields(kg_handle h, void **phi, void **T, void **theta)
     if (int rc = check_handle(h))
         return rc;
     invalidate_status(*h);  // the caller may write through these addresses
+    h->theta_given = HUGE_VAL;
     if (phi)
         *phi = h->phi[h->cur];
     if (T)

@@ -77,6 +77,7 @@ std::vector<kg::StepArgs<R>> member_args(const kg_batch_s &b)
     int tile_base = 0;
     for (int m = 0; m < b.count; ++m) {
         kg::StepArgs<R> a = kg::make_args<R>(b.params[m], b.nx[m], b.ny[m]);
+        a.wide_angles = kg::wide_angles(b.params[m], b.theta_given[m]);
         a.phi_in = reinterpret_cast<const R *>(b.field(b.phi[0], m));
         a.T_in = reinterpret_cast<const R *>(b.field(b.T[0], m));
         a.phi_out = reinterpret_cast<R *>(b.field(b.phi[1], m));
@@ -256,6 +257,7 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
     b->nx.assign(nx, nx + count);
     b->ny.assign(ny, ny + count);
     b->params.assign(params, params + count);
+    b->theta_given.assign(count, 0.0);
     const size_t align = 256 / b->esize();
     size_t total = 0;
     for (int m = 0; m < count; ++m) {
@@ -448,6 +450,20 @@ int kg_batch_set_fields(kg_batch b, int member, const void *phi, const void *T,
     else
         KG_HIP(hipMemsetAsync(b->field(b->theta, member), 0, bytes, b->stream));
     KG_HIP(hipStreamSynchronize(b->stream));
+    // the member's angle bound decides its sin/cos path: refresh its table entry
+    b->theta_given[member] = theta ? kg::max_abs(theta, b->cells(member), b->precision) : 0.0;
+    hipError_t e;
+    if (b->precision == KG_F64) {
+        const auto v = member_args<double>(*b);
+        e = hipMemcpy(static_cast<kg::StepArgs<double> *>(b->members) + member, &v[member],
+                      sizeof(v[member]), hipMemcpyHostToDevice);
+    } else {
+        const auto v = member_args<float>(*b);
+        e = hipMemcpy(static_cast<kg::StepArgs<float> *>(b->members) + member, &v[member],
+                      sizeof(v[member]), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "member table upload");
     return KG_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <cmath>
 #include <vector>
 
 #include "../../include/kobayashi_gpu.h"
@@ -39,6 +40,9 @@ struct kg_solver_s {
     uint32_t *stat = nullptr;
     int stat_tiles = 0, tiles_x = 0, tiles_y = 0;
     bool stat_ok = false, stat_valid = false;
+    // bound on |theta| of angles not written by the step (resets write 0,
+    // kg_set_fields what it is given; infinite after kg_device_fields)
+    double theta_given = 0.0;
 
     size_t cells() const { return (size_t)nx * (size_t)ny; }
     size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
@@ -78,6 +82,7 @@ struct kg_batch_s {
     uint32_t *stat = nullptr;
     int stat_tiles = 0;
     bool stat_valid = false;
+    std::vector<double> theta_given;  // per member, as kg_solver_s::theta_given
 
     size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
     size_t cells(int m) const { return (size_t)nx[m] * (size_t)ny[m]; }
@@ -126,6 +131,33 @@ StepArgs<R> make_args(const kg_params &p, int nx, int ny)
     a.key0 = (uint32_t)p.seed;
     a.key1 = (uint32_t)(p.seed >> 32);
     return a;
+}
+
+// |theta| the step itself writes stays below 2 pi + pi/2 < 8.
+constexpr double STEP_THETA_BOUND = 8.0;
+
+// StepArgs::wide_angles for parameters p and a bound on the stored |theta|:
+// whether |aniso (theta - theta0)| may leave the reduced sin/cos range (with
+// a margin for float rounding; NaN or infinite bounds count as wide).
+inline int wide_angles(const kg_params &p, double theta_given)
+{
+    const double b = theta_given > STEP_THETA_BOUND ? theta_given : STEP_THETA_BOUND;
+    const double m = std::fabs(p.aniso) * (b + std::fabs(p.theta0));
+    return m <= 0.99e4 ? 0 : 1;
+}
+
+// largest |v| of n host values of the given precision (NaN -> infinity)
+inline double max_abs(const void *v, size_t n, int precision)
+{
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double x = precision == KG_F64 ? static_cast<const double *>(v)[i]
+                                             : (double)static_cast<const float *>(v)[i];
+        const double a = std::fabs(x);
+        if (!(a <= m))
+            m = a == a ? a : HUGE_VAL;
+    }
+    return m;
 }
 
 // Centres of `count` Philox-drawn nuclei (kg_reset_seeds).

@@ -31,6 +31,9 @@ struct StepArgs {
     R eps_bar, delta, aniso, theta0, neg_ebar_j_delta;
     R alpha_pi, gamma, teq, K, noise_a;
     uint32_t step, instance, key0, key1;
+    // 1 when |aniso (theta - theta0)| may exceed the reduced sin/cos range
+    // for some stored angle (the host tracks a bound on |theta|)
+    int wide_angles;
     // Tile status words of the fused kernel (one per 64 x 30 tile, bits in
     // kg_step.hip): stat_in describes the input buffers, stat_out the output
     // buffers; the word of the tile at (x0, y0) sits at tile_base +

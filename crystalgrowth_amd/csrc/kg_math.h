@@ -52,15 +52,15 @@ __device__ __forceinline__ float kg_atan_ratio(float y, float x)
     return ((x < 0.0f) != (y < 0.0f)) ? -p : p;
 }
 
-// sin and cos of x: Cody-Waite reduction by pi/2 (three-part constant, one
-// fma per part), then polynomials on [-pi/4, pi/4].  |x| > 1e4 (never reached
-// by j (theta - theta0) with theta from the step) uses the library call.
-__device__ __forceinline__ void kg_sincos(float x, float *s, float *c)
+// Largest |x| kg_sincos_reduced serves (its three-part reduction stays exact
+// well beyond it); callers that cannot bound |x| use kg_sincos.
+#define KG_SINCOS_REDUCED_MAX 1.0e4f
+
+// sin and cos of x for |x| <= KG_SINCOS_REDUCED_MAX: Cody-Waite reduction by
+// pi/2 (three-part constant, one fma per part), then polynomials on
+// [-pi/4, pi/4].
+__device__ __forceinline__ void kg_sincos_reduced(float x, float *s, float *c)
 {
-    if (fabsf(x) > 1.0e4f) {
-        sincosf(x, s, c);
-        return;
-    }
     const float k = rintf(x * 0.636619772f);
     float r = fmaf(-k, 1.570796371e+00f, x);
     r = fmaf(-k, -4.371138829e-08f, r);
@@ -84,8 +84,23 @@ __device__ __forceinline__ void kg_sincos(float x, float *s, float *c)
     *c = ((q + 1) & 2) ? -cv : cv;
 }
 
+// sin and cos of any float x: kg_sincos_reduced, or the library call beyond
+// its range (never reached by j (theta - theta0) with theta from the step)
+__device__ __forceinline__ void kg_sincos(float x, float *s, float *c)
+{
+    if (fabsf(x) > KG_SINCOS_REDUCED_MAX) {
+        sincosf(x, s, c);
+        return;
+    }
+    kg_sincos_reduced(x, s, c);
+}
+
 __device__ __forceinline__ double kg_atan(double x) { return atan(x); }
 __device__ __forceinline__ double kg_atan_ratio(double y, double x) { return atan(y / x); }
 __device__ __forceinline__ void kg_sincos(double x, double *s, double *c) { sincos(x, s, c); }
+__device__ __forceinline__ void kg_sincos_reduced(double x, double *s, double *c)
+{
+    sincos(x, s, c);
+}
 
 }  // namespace kg

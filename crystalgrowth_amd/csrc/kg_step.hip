@@ -119,8 +119,9 @@ __device__ __forceinline__ int map_index(int x, int n)
     return x;
 }
 
-// The angle update of src/Kobayashi.cpp:154-167.  Returns true when a branch
-// fires (theta := th); false leaves the stored theta in force.  NaN
+// The angle update of src/Kobayashi.cpp:154-167 applied to th (the stored
+// angle on entry).  Returns true when a branch fires and rewrote th; false
+// leaves th, the stored angle, in force.  NaN
 // gradients fire nothing, as in the reference.  atan(gy / gx) is evaluated
 // only where a branch that uses it fires, so a wave whose cells all keep
 // their angle (flat gradients) or take the axis values skips it.
@@ -136,9 +137,8 @@ __device__ __forceinline__ bool theta_branch(R gx, R gy, R &th)
     if (with_atan) {
         const R t = kg_atan_ratio(gy, gx);  // atan(gy / gx)
         th = xneg ? PI + t : (yneg ? (R)2.0 * PI + t : t);
-    } else if (on_axis) {
-        th = yneg ? (R)-0.5 * PI : (R)0.5 * PI;
     }
+    th = on_axis ? (yneg ? (R)-0.5 * PI : (R)0.5 * PI) : th;  // the two never both hold
     return with_atan || on_axis;
 }
 
@@ -148,7 +148,12 @@ template <typename R>
 __device__ __forceinline__ void eps_of_angle(const StepArgs<R> &a, R th, R &eps, R &f)
 {
     R sn, cs;
-    kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
+    // wide_angles (uniform): some |j (theta - theta0)| may leave the reduced
+    // range, so every call checks; otherwise no per-lane branch
+    if (a.wide_angles)
+        kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
+    else
+        kg_sincos_reduced(a.aniso * (th - a.theta0), &sn, &cs);
     eps = a.eps_bar * ((R)1.0 + a.delta * cs);
     f = eps * (a.neg_ebar_j_delta * sn);
 }
@@ -312,9 +317,8 @@ __device__ __forceinline__ void pass1_cell(const StepArgs<R> &a, const ZeroAngle
 {
     const R gx = (z[c + 1] - z[c - 1]) * a.inv_dx;
     const R gy = (p[c] - m[c]) * a.inv_dy;
+    th = th_old;  // kept unless a branch fires
     fired = theta_branch(gx, gy, th);
-    if (!fired)
-        th = th_old;
     R eps, f;
     if (is_pos_zero(th)) {
         eps = z0.eps;
@@ -811,11 +815,9 @@ __global__ __launch_bounds__(256) void k_pass1(StepArgs<R> a, PassTemps<R> t)
     t.lapT[c] = ((R)2.0 * (at(T, ip, j) + at(T, im, j) + at(T, i, jp) + at(T, i, jm)) +
                  at(T, ip, jp) + at(T, im, jm) + at(T, im, jp) + at(T, ip, jm) -
                  (R)12.0 * at(T, i, j)) * a.inv_3dx2;
-    R th;
+    R th = a.theta[c];
     if (theta_branch(gx, gy, th))
         a.theta[c] = th;
-    else
-        th = a.theta[c];
     R sn, cs;
     kg_sincos(a.aniso * (th - a.theta0), &sn, &cs);
     t.eps[c] = a.eps_bar * ((R)1.0 + a.delta * cs);

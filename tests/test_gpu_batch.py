@@ -142,3 +142,22 @@ def test_resets_and_reduce(kb):
         with kb.Solver(*sizes[0], "f32", params[0]) as s:
             s.reset_seeds(3, 2.0, 1, 99)
             assert np.array_equal(b.phase(0), s.phase())
+
+
+def test_member_with_wide_stored_angles(kb):
+    """A member given angles up to 1e5 (outside the reduced sin/cos range)
+    takes the checked path and still matches a plain handle bitwise."""
+    nx, ny = 70, 40
+    p = kb.default_params(aniso=6.0, theta0=0.3, boundary=1, noise_a=0.01, seed=4)
+    rng = np.random.default_rng(5)
+    phi = rng.random((ny, nx)).astype(np.float32)
+    phi[:, nx // 2:] = 0.3
+    T = (rng.random((ny, nx)) * 0.5).astype(np.float32)
+    th = rng.uniform(-1e5, 1e5, (ny, nx)).astype(np.float32)
+    with kb.Batch([(64, 64), (nx, ny)], [kb.default_params(), p], "f32") as b:
+        b.set_fields(1, phi, T, th)
+        b.step(20)
+        got = b.fields(1)
+    ref = plain(kb, nx, ny, "f32", p, (phi, T, th), 20)
+    for x, y in zip(got, ref):
+        assert np.array_equal(x, y)

@@ -73,6 +73,9 @@ CASES = {
     "odd_sizes_periodic_noise": (37, 23, dict(delta=0.04, aniso=5.0, noise_a=0.02, seed=11, instance=2), "disk"),
     "tiny_zero_flux": (5, 4, dict(boundary=1), "random"),
     "random_periodic": (70, 33, dict(aniso=4.0, theta0=0.7), "random"),
+    # stored angles up to 1e5 where phi is flat (the stale branch keeps
+    # them): j (theta - theta0) leaves the reduced sin/cos range
+    "wide_angles_zero_flux": (68, 40, dict(aniso=6.0, theta0=0.3, boundary=1), "wide"),
 }
 
 
@@ -83,6 +86,11 @@ def make_state(O, name, dtype):
         s = O.init_plus(nx, ny)
     elif init == "disk":
         s = O.init_disk(nx, ny, nx // 2 - 3, ny // 2 + 2, 6.0)
+    elif init == "wide":
+        rng = np.random.default_rng(nx * 1000 + ny)
+        phi = rng.random((ny, nx))
+        phi[:, nx // 2:] = 0.3
+        s = O.State(phi, rng.random((ny, nx)) * 0.6, rng.uniform(-1e5, 1e5, (ny, nx)))
     else:
         rng = np.random.default_rng(nx * 1000 + ny)
         s = O.State(rng.random((ny, nx)), rng.random((ny, nx)) * 0.6, rng.uniform(-1, 7, (ny, nx)))
