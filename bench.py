@@ -23,6 +23,8 @@ apart).  Beside the headline it reports:
                plain solver per member (one handle and stream per member)
   adjoint   -- loss + gradients of 1000 steps on 512^2 (checkpoint every 50)
                vs the plain forward of the same steps
+  dense_kernel -- the same 4096^2 trajectory on the fused launch without tile
+               status (KG_KERNEL_FUSED_DENSE, bitwise the same results)
   reference_config -- the reference's configuration (300^2 FP64, 2000 steps)
                on the GPU and on the CPU oracle, compared cell by cell
 --calibrate-copy runs only a device copy of known size (for scaling the
@@ -63,6 +65,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=5000,
                     help="steps of the batched-vs-looped comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the same trajectory on the fused kernel without tile status")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--sync-every", type=int, default=0,
                     help="synchronise after every N timed steps (for counter-collection "
@@ -171,6 +175,23 @@ def reference_config(kb, cpu_steps: int):
             "max_abs_diff_phi": float(np.max(np.abs(gphi - ref.phi))),
             "max_abs_diff_T": float(np.max(np.abs(gT - ref.T))),
             "solid_fraction": float((gphi > 0.5).mean())}
+
+
+def dense_run(kb, size: int, nsteps: int):
+    """The same trajectory with KG_KERNEL_FUSED_DENSE: the fused launch with
+    no tile status (every tile loads and stores all of its fields; bitwise
+    the same results, tests/test_gpu_tile_status.py).  Shows how much of the
+    headline comes from skipping work the status proves unnecessary."""
+    with kb.Solver(size, size, "f32", kb.default_params(**WORKLOAD)) as s:
+        s.set_kernel("fused_dense")
+        s.reset_seeds(**SEEDS)
+        s.synchronize()
+        ms = s.step_timed(nsteps)
+    per = ms / nsteps / 1e3
+    ach = ALG_BYTES_FP32 * size * size / per / 1e9
+    return {"value": round(size * size / per / 1e9, 3), "unit": "Gcell-updates/s",
+            "avg_launch_us": round(per * 1e6, 3), "achieved_GBs": round(ach, 1),
+            "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
 def fp64_error(kb, size: int, nsteps: int = 100):
@@ -346,6 +367,8 @@ def main():
         "final_state": {"solid_fraction": st["solid_fraction"], "sum_phi": st["sum_phi"],
                         "nonfinite": st["nonfinite"]},
     }
+    if args.kernel == "fused" and not args.no_dense:
+        rec["dense_kernel"] = dense_run(kb, n, args.steps)
     if not args.no_fp64_check:
         rec["fp64_rel_err_100_steps"] = fp64_error(kb, n)
     if args.batch_steps > 0:
