@@ -253,49 +253,102 @@ __device__ __forceinline__ void fill_rows(R *__restrict__ dst, int rows, R v)
 }
 
 // Stage `rows` grid rows starting at y (mapped by the boundary) of field f,
-// columns x0-4 .. x0+67, into LDS rows of width LW.  With nx % 4 == 0 every
-// 4-column segment lies wholly inside or wholly outside the grid: inside it
-// is one aligned 16-byte load; outside, periodic wraps it onto another
-// aligned segment and zero-flux clamps it to one edge value.  Other widths
-// load cell by cell through the boundary map.  CHECK 1 accumulates phi's
-// flatness (all +0 / all 1) into f0/f1, CHECK 2 T's finiteness into f0.
+// columns x0-4 .. x0+67, into LDS rows of width LW, cell by cell through the
+// boundary map (widths nx % 4 != 0; other widths use issue_rows and
+// commit_rows below).  CHECK 1 accumulates phi's flatness (all +0 / all 1)
+// into f0/f1, CHECK 2 T's finiteness into f0.
 template <typename R, int BC, int CHECK>
 __device__ __forceinline__ void stage_rows(const R *__restrict__ f, R *__restrict__ dst, int nx,
                                            int ny, int x0, int y, int rows, int &f0, int &f1)
 {
-    const int tid = threadIdx.x;
+    for (int k = threadIdx.x; k < rows * LW; k += NT) {
+        const int r = k / LW, c = k - r * LW;
+        const R v = f[(size_t)map_index<BC>(y + r, ny) * nx + map_index<BC>(x0 - 4 + c, nx)];
+        dst[k] = v;
+        if (CHECK == 1) {
+            f0 &= is_pos_zero(v);
+            f1 &= (v == (R)1.0);
+        } else if (CHECK == 2) {
+            f0 &= isfinite(v) ? 1 : 0;
+        }
+    }
+}
+
+// Split staging for nx % 4 == 0.  Every 4-column segment then lies wholly
+// inside or wholly outside the grid: inside it is one aligned 16-byte load;
+// outside, periodic wraps it onto another aligned segment and zero-flux
+// repeats one edge value.  A field's ROWS x (LW / 4) segments are at most two
+// per thread (segment tid and tid + NT).  issue_rows loads a
+// thread's segments into registers and commit_rows writes them to LDS, so a
+// tile issues the loads of all of its fields before it waits on any: one
+// memory round trip for the tile instead of one per loop trip and field.
+template <typename R>
+struct Seg2 {
+    R a[4], b[4];
+    int ea, eb;  // zero-flux edge segment: 0 inside, 1 left (repeat a[0]), 2 right (a[3])
+};
+
+// Issue the load of segment k (no use of its data here, so the loads of a
+// tile's fields stay in flight together); returns its edge code.
+template <typename R, int BC>
+__device__ __forceinline__ int load_segment(const R *__restrict__ f, int nx, int ny, int x0, int y,
+                                            int k, R q[4], bool real = true)
+{
     constexpr int V = LW / 4;
-    if ((nx & 3) == 0) {
-        for (int k = tid; k < rows * V; k += NT) {
-            const int r = k / V, v = k - r * V;
-            const R *row = f + (size_t)map_index<BC>(y + r, ny) * nx;
-            const int gx = x0 - 4 + 4 * v;
-            R q[4];
-            if (gx >= 0 && gx < nx) {
-                load4(row + gx, q);
-            } else if (BC == 0) {
-                load4(row + map_index<0>(gx, nx), q);
-            } else {
-                q[0] = q[1] = q[2] = q[3] = row[gx < 0 ? 0 : nx - 1];
-            }
-            store4(&dst[r * LW + 4 * v], q);
-            if (CHECK == 1)
-                flat4(q, f0, f1);
-            else if (CHECK == 2)
-                f0 &= finite4(q);
-        }
-    } else {
-        for (int k = tid; k < rows * LW; k += NT) {
-            const int r = k / LW, c = k - r * LW;
-            const R v = f[(size_t)map_index<BC>(y + r, ny) * nx + map_index<BC>(x0 - 4 + c, nx)];
-            dst[k] = v;
-            if (CHECK == 1) {
-                f0 &= is_pos_zero(v);
-                f1 &= (v == (R)1.0);
-            } else if (CHECK == 2) {
-                f0 &= isfinite(v) ? 1 : 0;
-            }
-        }
+    const int r = k / V, v = k - r * V;
+    const R *row = f + (size_t)map_index<BC>(y + r, ny) * nx;
+    const int gx = x0 - 4 + 4 * v;
+    const bool inside = gx >= 0 && gx < nx;
+    // one aligned load in every case: periodic wraps the segment, zero-flux
+    // loads the edge segment, whose edge value commit_rows repeats
+    const int gl = inside ? gx : (BC == 0 ? map_index<0>(gx, nx) : (gx < 0 ? 0 : nx - 4));
+    load4(real ? row + gl : f, q);  // !real: an aligned dummy load
+    return (BC == 1 && !inside) ? (gx < 0 ? 1 : 2) : 0;
+}
+
+// real false: the same loads from f's first segment (results unused), for a
+// caller that must issue them without a branch
+template <typename R, int BC, int ROWS>
+__device__ __forceinline__ void issue_rows(const R *__restrict__ f, int nx, int ny, int x0, int y,
+                                           Seg2<R> &s, bool real = true)
+{
+    constexpr int N = ROWS * (LW / 4);
+    static_assert(N > NT && N <= 2 * NT, "two segments per thread");
+    const int tid = threadIdx.x;
+    s.ea = load_segment<R, BC>(f, nx, ny, x0, y, tid, s.a, real);
+    // threads without a second segment load the last one again (unused), so
+    // no branch separates the loads from the ones issued before them
+    s.eb = load_segment<R, BC>(f, nx, ny, x0, y, tid + NT < N ? tid + NT : N - 1, s.b, real);
+}
+
+template <typename R>
+__device__ __forceinline__ void fix_edge(R q[4], int e)
+{
+    const R v = e == 1 ? q[0] : q[3];
+    if (e != 0)
+        q[0] = q[1] = q[2] = q[3] = v;
+}
+
+// CHECK as in stage_rows
+template <typename R, int ROWS, int CHECK>
+__device__ __forceinline__ void commit_rows(Seg2<R> &s, R *__restrict__ dst, int &f0, int &f1)
+{
+    constexpr int V = LW / 4, N = ROWS * V;
+    const int tid = threadIdx.x;
+    fix_edge(s.a, s.ea);
+    store4(&dst[tid / V * LW + 4 * (tid % V)], s.a);
+    if (CHECK == 1)
+        flat4(s.a, f0, f1);
+    else if (CHECK == 2)
+        f0 &= finite4(s.a);
+    if (tid + NT < N) {
+        const int k = tid + NT;
+        fix_edge(s.b, s.eb);
+        store4(&dst[k / V * LW + 4 * (k % V)], s.b);
+        if (CHECK == 1)
+            flat4(s.b, f0, f1);
+        else if (CHECK == 2)
+            f0 &= finite4(s.b);
     }
 }
 
@@ -352,20 +405,26 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     uint32_t known = 0, has = 0;
     bool active = false;
     int sidx = 0;
+    // With the status, T is loaded right after the status words, before they
+    // are read: every tile but a quiescent one stages T, and the two loads
+    // then share one memory round trip.
+    const bool spec_T = track && (nx & 3) == 0;
+    Seg2<R> gt;
     if (track) {
         const int tx = x0 / TX, ty = y0 / TY;
         sidx = a.tile_base + ty * a.tiles_x + tx;
         // lanes 0..8 of every wave load the 3 x 3 neighbourhood's input
         // words (lane 4 = this tile), lane 9 this tile's output word
+        // (lanes past 9 load lane 9's word again: one load, no branch, so
+        // the wait below is for it alone and not for the T loads after it)
         const int lane = tid & 63;
-        uint32_t wv = 0u;
-        if (lane < 9) {
-            const int yy = map_tile<BC>(ty + lane / 3 - 1, a.tiles_y);
-            const int xx = map_tile<BC>(tx + lane % 3 - 1, a.tiles_x);
-            wv = a.stat_in[a.tile_base + yy * a.tiles_x + xx];
-        } else if (lane == 9) {
-            wv = a.stat_out[sidx];
-        }
+        const int ln = lane < 9 ? lane : 4;
+        const int yy = map_tile<BC>(ty + ln / 3 - 1, a.tiles_y);
+        const int xx = map_tile<BC>(tx + ln % 3 - 1, a.tiles_x);
+        const uint32_t wv =
+            lane < 9 ? a.stat_in[a.tile_base + yy * a.tiles_x + xx] : a.stat_out[sidx];
+        issue_rows<R, BC, QH>(a.T_in, nx, ny, x0, y0 - 1, gt, spec_T);
+        __builtin_amdgcn_sched_barrier(0);  // keep the T loads ahead of the wait
         const uint64_t nine = 0x1ffull;
         known = ((__ballot(lane < 9 && (wv & S_PHI0)) & nine) == nine ? S_PHI0 : 0u) |
                 ((__ballot(lane < 9 && (wv & S_PHI1)) & nine) == nine ? S_PHI1 : 0u) |
@@ -410,19 +469,38 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     // (a field known to be constant is not loaded; its LDS rows are filled
     // only if the full path needs them)
     const bool phi_known = (known & (S_PHI0 | S_PHI1)) != 0;
+    const bool T_known = (known & S_T0) != 0;
     int all0 = (known & S_PHI0) ? 1 : 0, all1 = (known & S_PHI1) ? 1 : 0, finite = 1;
-    if (!phi_known) {
+    if (!phi_known)
         all0 = all1 = 1;
-        stage_rows<R, BC, 1>(a.phi_in, sP, nx, ny, x0, y0 - 2, PH, all0, all1);
-    }
-    if (known & S_T0)
-        fill_rows(sT, QH, (R)0);
-    else
-        stage_rows<R, BC, 2>(a.T_in, sT, nx, ny, x0, y0 - 1, QH, finite, finite);
-    if (active) {
+    if ((nx & 3) == 0) {
+        Seg2<R> gp, gh;
+        if (!phi_known)
+            issue_rows<R, BC, PH>(a.phi_in, nx, ny, x0, y0 - 2, gp);
+        if (!spec_T && !T_known)
+            issue_rows<R, BC, QH>(a.T_in, nx, ny, x0, y0 - 1, gt);
+        if (active)
+            issue_rows<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
         int unused0 = 1, unused1 = 1;
-        stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+        if (!phi_known)
+            commit_rows<R, PH, 1>(gp, sP, all0, all1);
+        // a T known to be +0 was loaded as +0 (the status holds bitwise)
+        if (spec_T || !T_known)
+            commit_rows<R, QH, 2>(gt, sT, finite, finite);
+        if (active)
+            commit_rows<R, QH, 0>(gh, sB, unused0, unused1);
+    } else {
+        if (!phi_known)
+            stage_rows<R, BC, 1>(a.phi_in, sP, nx, ny, x0, y0 - 2, PH, all0, all1);
+        if (!T_known)
+            stage_rows<R, BC, 2>(a.T_in, sT, nx, ny, x0, y0 - 1, QH, finite, finite);
+        if (active) {
+            int unused0 = 1, unused1 = 1;
+            stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+        }
     }
+    if (T_known && !spec_T)
+        fill_rows(sT, QH, (R)0);
     // block-wide AND of the two flatness flags with one barrier: each wave
     // posts its ballots, every thread combines the eight
     {
@@ -443,7 +521,13 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         // ---- stage the old theta of the pass-1 cells into B ---------------
         if (!active) {
             int unused0 = 1, unused1 = 1;
-            stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+            if ((nx & 3) == 0) {
+                Seg2<R> gh;
+                issue_rows<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
+                commit_rows<R, QH, 0>(gh, sB, unused0, unused1);
+            } else {
+                stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+            }
         }
         if (phi_known)
             fill_rows(sP, PH, (known & S_PHI0) ? (R)0 : (R)1);
