@@ -167,6 +167,24 @@ __device__ __forceinline__ R lap9(const R *m, const R *z, const R *p, int c)
            m[c + 1] - (R)12.0 * z[c];
 }
 
+// End of a pass-2 phase: the four values (the phase's results) are taken to
+// exist in registers here and LDS reads may not move across, so the
+// compiler keeps a phase's arithmetic ahead of the next phase's window reads
+// and a phase's windows die before the next are read (no spills at the
+// register budget of 6 waves per SIMD).
+template <typename R>
+__device__ __forceinline__ void phase_end(R (&v)[4])
+{
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+        asm volatile("" : "+v"(v[e]));
+}
+__device__ __forceinline__ void phase_fence()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 // Pass-1 and pass-2 threads map one tile row to 16 consecutive lanes (one
 // DPP row).  A thread's left and right neighbour columns are then held by
 // the neighbouring lanes: row_shr:1 hands lane l the value of lane l-1,
@@ -648,6 +666,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             lapT[e] = lap9(Tm, Tz, Tp, e + 1) * a.inv_3dx2;
             Tc[e] = Tz[e + 1];
         }
+        phase_end(lapT);
+        phase_end(Tc);
+        phase_fence();
         R phi[4], lap[4], gx[4], gy[4];
         {
             R Pm[6], Pz[6], Pp[6];
@@ -663,6 +684,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 gy[e] = (Pp[c] - Pm[c]) * a.inv_dy;
             }
         }
+        phase_end(phi);
+        phase_end(lap);
+        phase_end(gx);
+        phase_end(gy);
+        phase_fence();
         R e2c[4], t3[4];
         {
             R E2m[4], E2z[6], E2p[4];
@@ -678,6 +704,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 t3[e] = gEx * gx[e] + gEy * gy[e];
             }
         }
+        phase_end(e2c);
+        phase_end(t3);
+        phase_fence();
         R t1[4], t2[4];
         {
             R Am[4], Ap[4], Bz[6];
@@ -690,6 +719,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 t2[e] = -(Bz[e + 2] - Bz[e]) * a.inv_dx;
             }
         }
+        phase_end(t1);
+        phase_end(t2);
+        phase_fence();
 
         // the reaction matters only where phi (1 - phi) != 0 (or T is not
         // finite, which the reference would turn into NaN)
