@@ -223,8 +223,9 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
 
 __device__ __forceinline__ int xcd_tile(int per_xcd)
 {
-    // Block b runs on XCD b % 8; give each XCD a contiguous run of tiles so
-    // that halo rows shared by neighbouring tiles stay in one L2.
+    // Block b runs on XCD b % 8; give each XCD a contiguous run of tiles (of
+    // a batch: of members) so that halos shared by neighbouring tiles stay in
+    // one L2.
     const int b = blockIdx.x;
     return (b % NXCD) * per_xcd + b / NXCD;
 }
@@ -822,7 +823,10 @@ template <typename R, int BC>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a)
 {
     __shared__ __align__(16) TileSmem<R> sm;
-    const int tile = xcd_tile(a.per_xcd);
+    // tiles in row-major order: block b runs on XCD b % 8, so each XCD takes
+    // every 8th tile column over the whole grid (balanced when the active
+    // tiles cluster), while the tiles above and below share its L2
+    const int tile = blockIdx.x;
     if (tile >= a.ntiles)
         return;
     const int nbx = (a.nx + TX - 1) / TX;
