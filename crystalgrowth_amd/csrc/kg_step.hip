@@ -454,7 +454,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             isfinite(a.inv_3dx2)) {
             // phi = +0 and T = +0 on the tile and its halos: the step leaves
             // both +0 (every term is a signed zero summing to +0); store only
-            // what the output buffers do not already hold
+            // what the output buffers do not already hold.  When the output
+            // word already reads exactly this status, nothing changes.
+            if (has == (S_PHI0 | S_T0))
+                return;
             const bool w_phi = !(has & S_PHI0), w_T = !(has & S_T0);
             if (w_phi || w_T) {
                 const int g = tid % NG, row = tid / NG;
