@@ -221,6 +221,43 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
     w[5] = rowp[kc + 4];
 }
 
+// Order of the tiles over the blocks of the single-grid kernel.  Block b
+// runs on XCD b % 8 (workgroups go round-robin over the XCDs) and takes the
+// (b / 8)-th tile of that XCD's sequence.  The grid is cut into chunks of
+// 2 x 8 tiles; chunks go round-robin over the XCDs in row-major chunk order,
+// each chunk row rotated by 3 chunks, and an XCD walks its chunks one after
+// the other, the 16 tiles of a chunk in row-major order.  So at any time
+// every XCD works on tiles spread over the grid (clustered dendrites load
+// the XCDs evenly, and cheap and expensive tiles run side by side), while
+// tiles that share halos are processed together on one XCD and meet in its
+// L2.  Measured on 20000 steps of a 4096 x 4096 grid growing nine seeds
+// (mean of 10 checkpoint steps): 55.1 us row-major, 55.8 us one band per
+// XCD, 46.0 us 2 x 8 chunks unrotated, 44.3 us rotated; permuting the chunk
+// order (49.7 - 51.5 us) loses the reuse between neighbouring chunks.
+constexpr int CHUNK_W = 2, CHUNK_H = 8, CHUNK_SKEW = 3;
+
+// Tile (tx, ty) of this block; false for blocks past the grid.
+__device__ __forceinline__ bool chunk_tile(int tiles_x, int tiles_y, int &tx, int &ty)
+{
+    const int b = blockIdx.x;
+    const int i = b / NXCD;
+    const int chunk = (i / (CHUNK_W * CHUNK_H)) * NXCD + b % NXCD;
+    const int k = i % (CHUNK_W * CHUNK_H);
+    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
+    const int cy = chunk / ncx;
+    const int cx = (chunk - cy * ncx + CHUNK_SKEW * cy) % ncx;  // a rotation of the row
+    tx = cx * CHUNK_W + k % CHUNK_W;
+    ty = cy * CHUNK_H + k / CHUNK_W;
+    return tx < tiles_x && ty < tiles_y;
+}
+
+// blocks of the single-grid launch: whole chunk rounds over the 8 XCDs
+int chunk_blocks(int tiles_x, int tiles_y)
+{
+    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
+    return (ncx * ncy + NXCD - 1) / NXCD * NXCD * CHUNK_W * CHUNK_H;
+}
+
 __device__ __forceinline__ int xcd_tile(int per_xcd)
 {
     // Block b runs on XCD b % 8; give each XCD a contiguous run of tiles (of
@@ -826,14 +863,10 @@ template <typename R, int BC>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a)
 {
     __shared__ __align__(16) TileSmem<R> sm;
-    // tiles in row-major order: block b runs on XCD b % 8, so each XCD takes
-    // every 8th tile column over the whole grid (balanced when the active
-    // tiles cluster), while the tiles above and below share its L2
-    const int tile = blockIdx.x;
-    if (tile >= a.ntiles)
+    int tx, ty;
+    if (!chunk_tile((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, tx, ty))
         return;
-    const int nbx = (a.nx + TX - 1) / TX;
-    step_tile<R, BC>(a, (tile % nbx) * TX, (tile / nbx) * TY, sm);
+    step_tile<R, BC>(a, tx * TX, ty * TY, sm);
 }
 
 // Batched step: every tile of every ensemble member in one launch.  tiles[t]
@@ -1149,6 +1182,7 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s)
     StepArgs<R> b = a;
     b.ntiles = ntiles;
     b.per_xcd = per;
+    nblocks = chunk_blocks((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY);
     if (a.boundary == 1)
         hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(nblocks), dim3(NT), 0, s, b);
     else
