@@ -353,8 +353,7 @@ int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out)
         return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of reduction scratch failed");
     }
     {
-        int nt, per, nb;
-        kg::fused_geometry(nx, ny, &nt, &per, &nb);
+        const int nt = kg::fused_tiles(nx, ny);
         h->stat_tiles = nt;
         h->tiles_x = (nx + kg::fused_tile_x() - 1) / kg::fused_tile_x();
         h->tiles_y = (ny + kg::fused_tile_y() - 1) / kg::fused_tile_y();

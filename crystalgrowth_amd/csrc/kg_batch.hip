@@ -3,8 +3,8 @@
 // All members live in one allocation per field (member offsets rounded up to
 // 256 bytes, so interior tiles keep their 16-byte vector loads).  A device
 // table holds each member's StepArgs; a tile table lists (member, x0, y0) for
-// every 64 x 30 tile of every member in member order, so the XCD-aware block
-// order hands each XCD a contiguous run of members.  One launch steps every
+// every 64 x 30 tile of every member, in the block order of batch_tile_table
+// (chunks of tiles dealt round-robin to the XCDs).  One launch steps every
 // tile of every member; KG_BATCH_GRAPH replays a captured hipGraph of an even
 // number of launches followed by a counter kernel, with the noise counter
 // kept on the device so replays draw fresh noise.
@@ -67,6 +67,46 @@ void free_batch(kg_batch_s *b)
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+// The tile table, in block order: block b steps the tile of entry b.  Every
+// member is cut into chunks of 2 x 8 tiles (chunk rows rotated by 3, as the
+// single-grid kernel orders its tiles); the chunks, member after member, are
+// dealt round-robin to the 8 XCDs (block b runs on XCD b % 8), each XCD
+// walks its chunks one after the other, and its sequence is padded to the
+// longest one with empty entries (member -1).  So every XCD steps tiles of
+// many members at any time, and tiles that share halos meet in one L2.
+// *real = number of tiles (entries that are not padding).
+std::vector<int4> batch_tile_table(const int *nx, const int *ny, int count, size_t *real)
+{
+    constexpr int NX = 8, CW = 2, CH = 8, SKEW = 3;
+    const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
+    std::vector<std::vector<int4>> seq(NX);
+    size_t chunk = 0, n = 0;
+    for (int m = 0; m < count; ++m) {
+        const int tx = (nx[m] + TX - 1) / TX, ty = (ny[m] + TY - 1) / TY;
+        const int ncx = (tx + CW - 1) / CW, ncy = (ty + CH - 1) / CH;
+        for (int cy = 0; cy < ncy; ++cy)
+            for (int c = 0; c < ncx; ++c, ++chunk) {
+                const int cx = (c + SKEW * cy) % ncx;
+                for (int k = 0; k < CW * CH; ++k) {
+                    const int x = cx * CW + k % CW, y = cy * CH + k / CW;
+                    if (x < tx && y < ty) {
+                        seq[chunk % NX].push_back(make_int4(m, x * TX, y * TY, 0));
+                        ++n;
+                    }
+                }
+            }
+    }
+    size_t len = 0;
+    for (const auto &q : seq)
+        len = std::max(len, q.size());
+    std::vector<int4> table(len * NX, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < NX; ++x)
+        for (size_t i = 0; i < seq[x].size(); ++i)
+            table[i * NX + x] = seq[x][i];
+    *real = n;
+    return table;
 }
 
 template <typename R>
@@ -265,12 +305,8 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
         total += (b->cells(m) + align - 1) / align * align;
     }
     b->total = total;
-    const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
-    std::vector<int4> tiles;
-    for (int m = 0; m < count; ++m)
-        for (int y0 = 0; y0 < ny[m]; y0 += TY)
-            for (int x0 = 0; x0 < nx[m]; x0 += TX)
-                tiles.push_back(make_int4(m, x0, y0, 0));
+    size_t real_tiles = 0;
+    std::vector<int4> tiles = batch_tile_table(nx, ny, count, &real_tiles);
     if (tiles.size() > (size_t)(1 << 30)) {
         delete b;
         return fail(KG_ERR_INVALID_ARGUMENT, "too many tiles");
@@ -298,9 +334,9 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
         }
     const size_t args_bytes = (size_t)count * (precision == KG_F64 ? sizeof(kg::StepArgs<double>)
                                                                     : sizeof(kg::StepArgs<float>));
-    b->stat_tiles = b->ntiles;
+    b->stat_tiles = (int)real_tiles;
     if (hipMalloc(&b->members, args_bytes) != hipSuccess ||
-        hipMalloc(&b->stat, 2 * tiles.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->stat, 2 * real_tiles * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&b->tiles, tiles.size() * sizeof(int4)) != hipSuccess ||
         hipMalloc(&b->step_base, sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&b->partials, kg::reduce_partials_size() * sizeof(double)) != hipSuccess ||

@@ -26,7 +26,6 @@ struct StepArgs {
     int nx, ny;
     int boundary;    // 0 periodic, 1 zero-flux
     int ntiles;      // tiles of the grid
-    int per_xcd;     // tiles per XCD slot of the block-id remap
     R inv_dx, inv_dy, inv_3dx2, dt, dt_tau;
     R eps_bar, delta, aniso, theta0, neg_ebar_j_delta;
     R alpha_pi, gamma, teq, K, noise_a;
@@ -84,8 +83,8 @@ template <typename R>
 hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double cy, int n,
                                 double level, double *radii, hipStream_t s);
 
-// Tiles of the fused kernel and its launch geometry.
-void fused_geometry(int nx, int ny, int *ntiles, int *per_xcd, int *nblocks);
+// Number of 64 x 30 tiles of the fused kernel on an nx x ny grid.
+int fused_tiles(int nx, int ny);
 // Tile status can be kept for an nx x ny grid with this boundary (every halo
 // of a tile lies in its 3 x 3 tile neighbourhood); fill `n` words with v.
 bool fused_status_ok(int nx, int ny, int boundary);

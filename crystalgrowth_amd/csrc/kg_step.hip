@@ -258,15 +258,6 @@ int chunk_blocks(int tiles_x, int tiles_y)
     return (ncx * ncy + NXCD - 1) / NXCD * NXCD * CHUNK_W * CHUNK_H;
 }
 
-__device__ __forceinline__ int xcd_tile(int per_xcd)
-{
-    // Block b runs on XCD b % 8; give each XCD a contiguous run of tiles (of
-    // a batch: of members) so that halos shared by neighbouring tiles stay in
-    // one L2.
-    const int b = blockIdx.x;
-    return (b % NXCD) * per_xcd + b / NXCD;
-}
-
 // Tile status: one word per tile and buffer, written by the block that
 // wrote the tile; a bit is set only when it holds on every cell of the tile.
 constexpr uint32_t S_PHI0 = 1u;    // phi is +0 (bit pattern)
@@ -878,15 +869,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 template <typename R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_batch(const StepArgs<R> *__restrict__ members,
                                                    const int4 *__restrict__ tiles, int ntiles,
-                                                   int per_xcd,
                                                    const uint32_t *__restrict__ step_base,
                                                    uint32_t step_add)
 {
     __shared__ __align__(16) TileSmem<R> sm;
-    const int t = xcd_tile(per_xcd);
+    const int t = blockIdx.x;
     if (t >= ntiles)
         return;
     const int4 d = tiles[t];
+    if (d.x < 0)
+        return;  // padding of the table
     StepArgs<R> a = members[d.x];
     a.step = *step_base + step_add;
     if (a.step & 1u) {
@@ -910,13 +902,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 template <typename R>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_batch_direct(
     const StepArgs<R> *__restrict__ members, const int4 *__restrict__ tiles, int ntiles,
-    int per_xcd, uint32_t step)
+    uint32_t step)
 {
     __shared__ __align__(16) TileSmem<R> sm;
-    const int t = xcd_tile(per_xcd);
+    const int t = blockIdx.x;
     if (t >= ntiles)
         return;
     const int4 d = tiles[t];
+    if (d.x < 0)
+        return;  // padding of the table
     StepArgs<R> a = members[d.x];
     a.step = step;
     a.stat_in = nullptr;  // the buffers are not the ones the status describes
@@ -1165,24 +1159,14 @@ unsigned grid_1d(size_t n, unsigned threads)
 
 }  // namespace
 
-void fused_geometry(int nx, int ny, int *ntiles, int *per_xcd, int *nblocks)
-{
-    const int tiles = ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY);
-    const int per = (tiles + NXCD - 1) / NXCD;
-    *ntiles = tiles;
-    *per_xcd = per;
-    *nblocks = per * NXCD;
-}
+int fused_tiles(int nx, int ny) { return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY); }
 
 template <typename R>
 hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s)
 {
-    int ntiles, per, nblocks;
-    fused_geometry(a.nx, a.ny, &ntiles, &per, &nblocks);
     StepArgs<R> b = a;
-    b.ntiles = ntiles;
-    b.per_xcd = per;
-    nblocks = chunk_blocks((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY);
+    b.ntiles = fused_tiles(a.nx, a.ny);
+    const int nblocks = chunk_blocks((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY);
     if (a.boundary == 1)
         hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(nblocks), dim3(NT), 0, s, b);
     else
@@ -1223,9 +1207,8 @@ template <typename R>
 hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
                              const uint32_t *step_base, uint32_t step_add, hipStream_t s)
 {
-    const int per = (ntiles + NXCD - 1) / NXCD;
-    hipLaunchKernelGGL((k_step_batch<R>), dim3(per * NXCD), dim3(NT), 0, s, members, tiles,
-                       ntiles, per, step_base, step_add);
+    hipLaunchKernelGGL((k_step_batch<R>), dim3(ntiles), dim3(NT), 0, s, members, tiles, ntiles,
+                       step_base, step_add);
     return hipGetLastError();
 }
 
@@ -1233,9 +1216,8 @@ template <typename R>
 hipError_t launch_step_batch_direct(const StepArgs<R> *members, const int4 *tiles, int ntiles,
                                     uint32_t step, hipStream_t s)
 {
-    const int per = (ntiles + NXCD - 1) / NXCD;
-    hipLaunchKernelGGL((k_step_batch_direct<R>), dim3(per * NXCD), dim3(NT), 0, s, members, tiles,
-                       ntiles, per, step);
+    hipLaunchKernelGGL((k_step_batch_direct<R>), dim3(ntiles), dim3(NT), 0, s, members, tiles,
+                       ntiles, step);
     return hipGetLastError();
 }
 
