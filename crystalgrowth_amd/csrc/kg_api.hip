@@ -7,6 +7,8 @@
 // so a kernel never sees a size or pointer it was not written for.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -96,6 +98,46 @@ int ensure_status(kg_solver_s &h)
         return KG_OK;
     KG_HIP(kg::launch_fill_u32(h.stat, 2 * (size_t)h.stat_tiles, 0u, h.stream));
     h.stat_valid = true;
+    // liveness answers about the old fields are void
+    h.live.n = -1;
+    h.live_pending[0] = h.live_pending[1] = false;
+    return KG_OK;
+}
+
+// Chunks the fused step works on.  At every step that is a multiple of
+// SKIP_PERIOD the host (1) takes the liveness answer it asked for two
+// periods ago, if any, as the list of chunks the next SKIP_PERIOD launches
+// cover (no answer, or more live chunks than a LiveList holds: all chunks),
+// and (2) asks again: k_chunk_live reads the
+// status as the stream stands and writes its flags to pinned host memory,
+// behind an event.  A chunk is skipped when every tile within
+// SKIP_MARGIN_TILES tiles of it is exactly phi = +0, T = +0 on both buffers
+// with status words saying so; nonzero values cannot reach the chunk in the
+// 3 * SKIP_PERIOD steps an answer is used for (kg_step.hip checks the
+// margin), and a step of such a tile would write the same zeros and status
+// the buffers hold already, so skipping changes no bit.  Waiting on an
+// answer bounds how far the host runs ahead of the GPU to about two periods.
+int skip_cycle(kg_solver_s &h)
+{
+    const int p = (int)((h.steps / kg::SKIP_PERIOD) & 1);
+    h.live.n = -1;
+    if (h.live_pending[p]) {
+        KG_HIP(hipEventSynchronize(h.live_ev[p]));
+        h.live_order.clear();
+        kg::live_chunk_order(h.nx, h.ny, h.live_host[p], h.live_order);
+        if (h.live_order.size() <= (size_t)kg::LIVE_MAX) {
+            std::copy(h.live_order.begin(), h.live_order.end(), h.live.c);
+            h.live.n = (int)h.live_order.size();
+        }
+        h.live_pending[p] = false;
+    }
+    // the kernel writes the flags straight into the pinned host buffer (a
+    // copy on the stream would hold the next step back)
+    KG_HIP(kg::launch_chunk_live(h.stat + (size_t)h.cur * h.stat_tiles,
+                                 h.stat + (size_t)(h.cur ^ 1) * h.stat_tiles, h.tiles_x,
+                                 h.tiles_y, h.params.boundary, h.live_host[p], h.stream));
+    KG_HIP(hipEventRecord(h.live_ev[p], h.stream));
+    h.live_pending[p] = true;
     return KG_OK;
 }
 
@@ -118,7 +160,11 @@ int step_impl(kg_solver_s &h, int nsteps)
         invalidate_status(h);  // the step writes the buffers without keeping the status
     else if (int rc = ensure_status(h))
         return rc;
+    const bool skipping = h.kernel == KG_KERNEL_FUSED && h.stat_ok && h.live_host[1] != nullptr;
     for (int s = 0; s < nsteps; ++s) {
+        if (skipping && h.steps % kg::SKIP_PERIOD == 0)
+            if (int rc = skip_cycle(h))
+                return rc;
         kg::StepArgs<R> a = make_args<R>(h);
         a.step = (uint32_t)h.steps;
         hipError_t e;
@@ -129,7 +175,7 @@ int step_impl(kg_solver_s &h, int nsteps)
                                base + 5 * n};
             e = kg::launch_step_two_pass<R>(a, t, h.stream);
         } else {
-            e = kg::launch_step_fused<R>(a, h.stream);
+            e = kg::launch_step_fused<R>(a, h.stream, skipping ? &h.live : nullptr);
         }
         if (e != hipSuccess)
             return hip_fail(e, "step launch");
@@ -186,6 +232,12 @@ void free_all(kg_solver_s *h)
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
+    for (int p = 0; p < 2; ++p) {
+        if (h->live_host[p])
+            (void)hipHostFree(h->live_host[p]);
+        if (h->live_ev[p])
+            (void)hipEventDestroy(h->live_ev[p]);
+    }
     if (h->ev0)
         (void)hipEventDestroy(h->ev0);
     if (h->ev1)
@@ -362,6 +414,24 @@ int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out)
             h->stat = nullptr;
             free_all(h);
             return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the tile status failed");
+        }
+        // chunk lists: chunk indices must fit 16 bits
+        const int nc = kg::fused_chunks_x(nx) * kg::fused_chunks_y(ny);
+        if (nc <= 65535) {
+            h->nchunks = nc;
+            bool ok = true;
+            for (int q = 0; q < 2 && ok; ++q) {
+                ok = hipHostMalloc(reinterpret_cast<void **>(&h->live_host[q]), (size_t)nc,
+                                   hipHostMallocDefault) == hipSuccess;
+                if (!ok)
+                    h->live_host[q] = nullptr;
+                ok = ok && hipEventCreateWithFlags(&h->live_ev[q], hipEventDisableTiming) ==
+                               hipSuccess;
+            }
+            if (!ok) {
+                free_all(h);
+                return fail(KG_ERR_OUT_OF_MEMORY, "allocation of the chunk skip state failed");
+            }
         }
     }
     rc = kg_reset(h);

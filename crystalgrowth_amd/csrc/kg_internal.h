@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace kg {
 
 // Everything one explicit-Euler step needs, with the reference's scalar
@@ -55,8 +57,36 @@ struct ReduceOut {
     double nonfinite;
 };
 
+// The chunks of 2 x 8 tiles that one launch of the fused step works on,
+// when not all: n chunk indices (row-major chunk order, c = cy * chunks_x +
+// cx) in the order the launch deals them to the XCDs.  Every chunk left out
+// holds phi = +0 and T = +0 on both buffers and keeps them over the launch,
+// and its status words already say so (the host derives the list from
+// launch_chunk_live; kg_api.hip states the conditions).  Passed by value in
+// the kernel arguments: the launch has only the listed chunks' blocks.
+constexpr int LIVE_MAX = 1020;
+struct LiveList {
+    int n;
+    uint16_t c[LIVE_MAX];
+};
+// live == nullptr or live->n < 0: every chunk
 template <typename R>
-hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s);
+hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live = nullptr);
+// Append to out the chunks flagged in flags[c] (one byte per chunk), in the
+// order a LiveList must hold them.
+void live_chunk_order(int nx, int ny, const uint8_t *flags, std::vector<uint16_t> &out);
+// Chunk grid of the fused step for an nx x ny grid (chunks_x * chunks_y chunks).
+int fused_chunks_x(int nx);
+int fused_chunks_y(int ny);
+// Tile rows/columns of margin around a chunk that launch_chunk_live checks,
+// and the period (steps) of the host's liveness requests.
+constexpr int SKIP_MARGIN_TILES = 2;
+constexpr int SKIP_PERIOD = 8;
+// live[c] = 1 if some tile within SKIP_MARGIN_TILES tiles of chunk c (mapped
+// by the boundary) has a status word other than exactly "phi = +0, T = +0"
+// in either set (stat_a, stat_b); 0 otherwise.
+hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,
+                             int tiles_y, int boundary, uint8_t *live, hipStream_t s);
 template <typename R>
 hipError_t launch_step_two_pass(const StepArgs<R> &a, const PassTemps<R> &t, hipStream_t s);
 

@@ -29,6 +29,7 @@
 #include <float.h>
 
 #include <algorithm>
+#include <vector>
 #include <math.h>
 
 #include "kg_internal.h"
@@ -236,8 +237,9 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
 // order (49.7 - 51.5 us) loses the reuse between neighbouring chunks.
 constexpr int CHUNK_W = 2, CHUNK_H = 8, CHUNK_SKEW = 3;
 
-// Tile (tx, ty) of this block; false for blocks past the grid.
-__device__ __forceinline__ bool chunk_tile(int tiles_x, int tiles_y, int &tx, int &ty)
+// Tile (tx, ty) of this block and its chunk's row-major index c; false for
+// blocks past the grid.
+__device__ __forceinline__ bool chunk_tile(int tiles_x, int tiles_y, int &tx, int &ty, int &c)
 {
     const int b = blockIdx.x;
     const int i = b / NXCD;
@@ -248,6 +250,7 @@ __device__ __forceinline__ bool chunk_tile(int tiles_x, int tiles_y, int &tx, in
     const int cx = (chunk - cy * ncx + CHUNK_SKEW * cy) % ncx;  // a rotation of the row
     tx = cx * CHUNK_W + k % CHUNK_W;
     ty = cy * CHUNK_H + k / CHUNK_W;
+    c = cy * ncx + cx;
     return tx < tiles_x && ty < tiles_y;
 }
 
@@ -850,14 +853,94 @@ template <typename R> struct Occupancy;
 template <> struct Occupancy<float> { static constexpr int waves = KG_WAVES_PER_EU; };
 template <> struct Occupancy<double> { static constexpr int waves = 1; };
 
+// Tile of this block when the launch covers only the chunks of `live`: the
+// j-th listed chunk goes to XCD j % 8 and its 16 tiles to consecutive
+// blocks of that XCD, as chunk_tile deals all chunks.
+__device__ __forceinline__ bool listed_tile(const LiveList &live, int tiles_x, int tiles_y, int &tx,
+                                            int &ty)
+{
+    const int b = blockIdx.x;
+    const int i = b / NXCD;
+    const int j = (i / (CHUNK_W * CHUNK_H)) * NXCD + b % NXCD;
+    if (j >= live.n)
+        return false;
+    const int c = live.c[j];
+    const int k = i % (CHUNK_W * CHUNK_H);
+    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
+    tx = (c % ncx) * CHUNK_W + k % CHUNK_W;
+    ty = (c / ncx) * CHUNK_H + k / CHUNK_W;
+    return tx < tiles_x && ty < tiles_y;
+}
+
 template <typename R, int BC>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a)
 {
     __shared__ __align__(16) TileSmem<R> sm;
-    int tx, ty;
-    if (!chunk_tile((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, tx, ty))
+    const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+    int tx, ty, c;
+    if (!chunk_tile(tiles_x, tiles_y, tx, ty, c))
         return;
     step_tile<R, BC>(a, tx * TX, ty * TY, sm);
+}
+
+// The same step over the chunks of `live` only (a separate kernel, so that
+// launches of every chunk keep their small kernel arguments).
+template <typename R, int BC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused_listed(StepArgs<R> a, LiveList live)
+{
+    __shared__ __align__(16) TileSmem<R> sm;
+    const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+    int tx, ty;
+    if (!listed_tile(live, tiles_x, tiles_y, tx, ty))
+        return;
+    step_tile<R, BC>(a, tx * TX, ty * TY, sm);
+}
+
+// A chunk skipped by a launch must stay all +0 for the launch.  The host uses
+// a liveness answer for the SKIP_PERIOD steps that start two periods after
+// it asked, so up to 3 SKIP_PERIOD steps separate the status it saw from the
+// last step that uses it.  A step moves nonzero values at most 2 cells
+// (pass 2 reads pass-1 values of neighbours, which read phi one cell
+// further), and a tile reads 2 cells beyond itself; the margin must cover
+// both.
+static_assert(2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_TILES * TY &&
+                  2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_TILES * TX,
+              "chunk skip margin too small for the request period");
+
+// Liveness of the chunks (see launch_chunk_live): one block per chunk, one
+// thread per tile of the chunk and its margin, all status loads in flight
+// together.
+constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_TILES, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_TILES;
+constexpr int LIVE_THREADS = (LIVE_W * LIVE_H + 63) / 64 * 64;
+__global__ __launch_bounds__(LIVE_THREADS) void k_chunk_live(const uint32_t *__restrict__ sa,
+                                                             const uint32_t *__restrict__ sb,
+                                                             int tiles_x, int tiles_y,
+                                                             int boundary,
+                                                             uint8_t *__restrict__ live)
+{
+    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
+    const int c = blockIdx.x;
+    const int cx = c % ncx, cy = c / ncx;
+    constexpr uint32_t Q = S_PHI0 | S_T0;
+    int lv = 0;
+    const int k = threadIdx.x;
+    if (k < LIVE_W * LIVE_H) {
+        const int y = cy * CHUNK_H - SKIP_MARGIN_TILES + k / LIVE_W;
+        const int x = cx * CHUNK_W - SKIP_MARGIN_TILES + k % LIVE_W;
+        int yy, xx;
+        if (boundary == 1) {
+            yy = y < 0 ? 0 : (y >= tiles_y ? tiles_y - 1 : y);
+            xx = x < 0 ? 0 : (x >= tiles_x ? tiles_x - 1 : x);
+        } else {
+            yy = ((y % tiles_y) + tiles_y) % tiles_y;
+            xx = ((x % tiles_x) + tiles_x) % tiles_x;
+        }
+        const int t = yy * tiles_x + xx;
+        lv = (sa[t] != Q) | (sb[t] != Q);
+    }
+    lv = __syncthreads_or(lv);
+    if (k == 0)
+        live[c] = lv ? 1 : 0;
 }
 
 // Batched step: every tile of every ensemble member in one launch.  tiles[t]
@@ -1162,15 +1245,54 @@ unsigned grid_1d(size_t n, unsigned threads)
 int fused_tiles(int nx, int ny) { return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY); }
 
 template <typename R>
-hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s)
+hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live)
 {
     StepArgs<R> b = a;
     b.ntiles = fused_tiles(a.nx, a.ny);
-    const int nblocks = chunk_blocks((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY);
+    const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+    const int nchunks = ((tiles_x + CHUNK_W - 1) / CHUNK_W) * ((tiles_y + CHUNK_H - 1) / CHUNK_H);
+    if (live && live->n >= 0 && live->n < nchunks) {
+        const int nblocks = (live->n + NXCD - 1) / NXCD * NXCD * CHUNK_W * CHUNK_H;
+        if (nblocks == 0)
+            return hipSuccess;  // no chunk to step
+        if (a.boundary == 1)
+            hipLaunchKernelGGL((k_step_fused_listed<R, 1>), dim3(nblocks), dim3(NT), 0, s, b, *live);
+        else
+            hipLaunchKernelGGL((k_step_fused_listed<R, 0>), dim3(nblocks), dim3(NT), 0, s, b, *live);
+        return hipGetLastError();
+    }
+    const int nblocks = chunk_blocks(tiles_x, tiles_y);
     if (a.boundary == 1)
         hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(nblocks), dim3(NT), 0, s, b);
     else
         hipLaunchKernelGGL((k_step_fused<R, 0>), dim3(nblocks), dim3(NT), 0, s, b);
+    return hipGetLastError();
+}
+
+// The chunk indices of the chunks flagged live, in the order chunk_tile
+// deals all chunks (chunk rows rotated by CHUNK_SKEW): appended to out.
+void live_chunk_order(int nx, int ny, const uint8_t *flags, std::vector<uint16_t> &out)
+{
+    const int tiles_x = (nx + TX - 1) / TX, tiles_y = (ny + TY - 1) / TY;
+    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
+    for (int chunk = 0; chunk < ncx * ncy; ++chunk) {
+        const int cy = chunk / ncx;
+        const int cx = (chunk - cy * ncx + CHUNK_SKEW * cy) % ncx;
+        const int c = cy * ncx + cx;
+        if (flags[c])
+            out.push_back((uint16_t)c);
+    }
+}
+
+int fused_chunks_x(int nx) { return ((nx + TX - 1) / TX + CHUNK_W - 1) / CHUNK_W; }
+int fused_chunks_y(int ny) { return ((ny + TY - 1) / TY + CHUNK_H - 1) / CHUNK_H; }
+
+hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,
+                             int tiles_y, int boundary, uint8_t *live, hipStream_t s)
+{
+    const int n = ((tiles_x + CHUNK_W - 1) / CHUNK_W) * ((tiles_y + CHUNK_H - 1) / CHUNK_H);
+    hipLaunchKernelGGL(k_chunk_live, dim3(n), dim3(LIVE_THREADS), 0, s, stat_a, stat_b, tiles_x,
+                       tiles_y, boundary, live);
     return hipGetLastError();
 }
 
@@ -1329,7 +1451,7 @@ hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double 
 }
 
 #define KG_INSTANTIATE(R)                                                                        \
-    template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t);                  \
+    template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t, const LiveList *); \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
                                              const uint32_t *, uint32_t, hipStream_t);           \
     template hipError_t launch_step_batch_direct<R>(const StepArgs<R> *, const int4 *, int,      \

@@ -120,3 +120,40 @@ def test_scalar_staging_and_tiny_grids(kb, boundary):
                           (5, 4, lambda s: s.reset())):
         pairs = run_pair(kb, nx, ny, "f32", p, setup, [1, 3, 60])
         assert_bitwise(pairs)
+
+
+@pytest.mark.parametrize("boundary,prec", [(1, "f32"), (0, "f32"), (1, "f64")])
+def test_chunk_skipping_matches_dense_bitwise(kb, boundary, prec):
+    """Far-field chunks are left alone by the status launch (their 2-tile
+    neighbourhood is all +0) while one nucleus grows into them.  Periodic:
+    the nucleus sits at a corner, so its fields wrap to all four edges.
+    Steps are taken in uneven calls, so the host's request periods fall
+    across call boundaries."""
+    nx, ny = 1024, 960         # 16 x 32 tiles, 8 x 4 chunks
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                          boundary=boundary, seed=4)
+    cx, cy = (3, 2) if boundary == 0 else (150, 140)
+    pairs = run_pair(kb, nx, ny, prec, p, lambda s: s.reset_disk(cx, cy, 6.0),
+                     [1, 6, 13, 31, 90, 301, 700])
+    assert_bitwise(pairs)
+
+
+def test_chunk_skipping_after_new_fields(kb):
+    """A blob written far from the old activity while skip answers about the
+    old fields are still in flight: the answers must be dropped."""
+    nx, ny = 1024, 960
+    p = kb.default_params(noise_a=0.01, boundary=1, seed=6)
+    with kb.Solver(nx, ny, "f32", p) as a, kb.Solver(nx, ny, "f32", p) as b:
+        b.set_kernel("fused_dense")
+        for s in (a, b):
+            s.reset_disk(100, 100, 5.0)
+            s.step(41)
+        phi = np.zeros((ny, nx), np.float32)
+        phi[800:812, 900:915] = 1.0
+        T = np.zeros((ny, nx), np.float32)
+        T[700:720, 20:60] = 0.2
+        for s in (a, b):
+            s.set_fields(phi, T)
+            s.step(3)
+            s.step(30)
+        assert_bitwise([(74, a.fields(), b.fields())])
