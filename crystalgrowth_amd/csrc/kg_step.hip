@@ -371,6 +371,22 @@ __device__ __forceinline__ void issue_rows(const R *__restrict__ f, int nx, int 
     s.eb = load_segment<R, BC>(f, nx, ny, x0, y, tid + NT < N ? tid + NT : N - 1, s.b, real);
 }
 
+// The same, with the second segment issued only by waves that have one
+// (a wave-uniform branch: the other waves skip its address arithmetic);
+// for loads whose wait may cover every load issued so far.
+template <typename R, int BC, int ROWS>
+__device__ __forceinline__ void issue_rows_w(const R *__restrict__ f, int nx, int ny, int x0, int y,
+                                             Seg2<R> &s)
+{
+    constexpr int N = ROWS * (LW / 4);
+    static_assert(N > NT && N <= 2 * NT, "two segments per thread");
+    const int tid = threadIdx.x;
+    s.ea = load_segment<R, BC>(f, nx, ny, x0, y, tid, s.a);
+    s.eb = 0;
+    if ((tid & ~63) + NT < N)
+        s.eb = load_segment<R, BC>(f, nx, ny, x0, y, tid + NT < N ? tid + NT : N - 1, s.b);
+}
+
 template <typename R>
 __device__ __forceinline__ void fix_edge(R q[4], int e)
 {
@@ -529,11 +545,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     if ((nx & 3) == 0) {
         Seg2<R> gp, gh;
         if (!phi_known)
-            issue_rows<R, BC, PH>(a.phi_in, nx, ny, x0, y0 - 2, gp);
+            issue_rows_w<R, BC, PH>(a.phi_in, nx, ny, x0, y0 - 2, gp);
         if (!spec_T && !T_known)
-            issue_rows<R, BC, QH>(a.T_in, nx, ny, x0, y0 - 1, gt);
+            issue_rows_w<R, BC, QH>(a.T_in, nx, ny, x0, y0 - 1, gt);
         if (active)
-            issue_rows<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
+            issue_rows_w<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
         int unused0 = 1, unused1 = 1;
         if (!phi_known)
             commit_rows<R, PH, 1>(gp, sP, all0, all1);
@@ -576,7 +592,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             int unused0 = 1, unused1 = 1;
             if ((nx & 3) == 0) {
                 Seg2<R> gh;
-                issue_rows<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
+                issue_rows_w<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
                 commit_rows<R, QH, 0>(gh, sB, unused0, unused1);
             } else {
                 stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
