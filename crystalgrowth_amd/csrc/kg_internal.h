@@ -67,8 +67,9 @@ struct ReduceOut {
 constexpr int LIVE_MAX = 1020;
 struct LiveList {
     int n;
-    uint16_t c[LIVE_MAX];
+    alignas(4) uint16_t c[LIVE_MAX];  // read by the kernel as 32-bit words
 };
+static_assert(LIVE_MAX % 2 == 0, "LiveList entries are read in pairs");
 // live == nullptr or live->n < 0: every chunk
 template <typename R>
 hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live = nullptr);

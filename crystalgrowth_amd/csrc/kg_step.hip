@@ -340,20 +340,29 @@ struct Seg2 {
 
 // Issue the load of segment k (no use of its data here, so the loads of a
 // tile's fields stay in flight together); returns its edge code.
-template <typename R, int BC>
-__device__ __forceinline__ int load_segment(const R *__restrict__ f, int nx, int ny, int x0, int y,
-                                            int k, R q[4], bool real = true)
+// Element offset of segment k in the field and its edge code.
+template <int BC>
+__device__ __forceinline__ size_t segment_offset(int nx, int ny, int x0, int y, int k, int &edge)
 {
     constexpr int V = LW / 4;
     const int r = k / V, v = k - r * V;
-    const R *row = f + (size_t)map_index<BC>(y + r, ny) * nx;
     const int gx = x0 - 4 + 4 * v;
     const bool inside = gx >= 0 && gx < nx;
     // one aligned load in every case: periodic wraps the segment, zero-flux
     // loads the edge segment, whose edge value commit_rows repeats
     const int gl = inside ? gx : (BC == 0 ? map_index<0>(gx, nx) : (gx < 0 ? 0 : nx - 4));
-    load4(real ? row + gl : f, q);  // !real: an aligned dummy load
-    return (BC == 1 && !inside) ? (gx < 0 ? 1 : 2) : 0;
+    edge = (BC == 1 && !inside) ? (gx < 0 ? 1 : 2) : 0;
+    return (size_t)map_index<BC>(y + r, ny) * nx + gl;
+}
+
+template <typename R, int BC>
+__device__ __forceinline__ int load_segment(const R *__restrict__ f, int nx, int ny, int x0, int y,
+                                            int k, R q[4], bool real = true)
+{
+    int edge;
+    const size_t o = segment_offset<BC>(nx, ny, x0, y, k, edge);
+    load4(real ? f + o : f, q);  // !real: an aligned dummy load
+    return edge;
 }
 
 // real false: the same loads from f's first segment (results unused), for a
@@ -366,9 +375,14 @@ __device__ __forceinline__ void issue_rows(const R *__restrict__ f, int nx, int 
     static_assert(N > NT && N <= 2 * NT, "two segments per thread");
     const int tid = threadIdx.x;
     s.ea = load_segment<R, BC>(f, nx, ny, x0, y, tid, s.a, real);
-    // threads without a second segment load the last one again (unused), so
-    // no branch separates the loads from the ones issued before them
-    s.eb = load_segment<R, BC>(f, nx, ny, x0, y, tid + NT < N ? tid + NT : N - 1, s.b, real);
+    // the second segment's load is issued by every thread (no branch around
+    // a load separates it from the loads before it); only waves that have
+    // second segments compute their addresses, the others load f[0..3]
+    size_t ob = 0;
+    s.eb = 0;
+    if ((tid & ~63) + NT < N)
+        ob = segment_offset<BC>(nx, ny, x0, y, tid + NT < N ? tid + NT : N - 1, s.eb);
+    load4(real ? f + ob : f, s.b);
 }
 
 // The same, with the second segment issued only by waves that have one
@@ -880,7 +894,9 @@ __device__ __forceinline__ bool listed_tile(const LiveList &live, int tiles_x, i
     const int j = (i / (CHUNK_W * CHUNK_H)) * NXCD + b % NXCD;
     if (j >= live.n)
         return false;
-    const int c = live.c[j];
+    // the 32-bit word holding entry j (a scalar load; there is no 16-bit one)
+    const uint32_t w = reinterpret_cast<const uint32_t *>(live.c)[j >> 1];
+    const int c = (int)((j & 1) ? w >> 16 : w & 0xffffu);
     const int k = i % (CHUNK_W * CHUNK_H);
     const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
     tx = (c % ncx) * CHUNK_W + k % CHUNK_W;
