@@ -13,7 +13,7 @@ the configuration's full horizon), with the fields resident in HBM.
 Prints one JSON line.  `value` is cell-updates per second of the timed
 region; `roofline` prices the step's average duration on the GPU (HIP events
 on the solver's own stream around the timed steps: each step is one launch
-of the fused kernel, over every 2 x 8-tile chunk or over the live ones, plus
+of the fused kernel, over every 2 x 4-tile chunk or over the live ones, plus
 the chunk liveness kernel every 8 steps) at 16 algorithmic bytes per
 cell-update against the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the reference's
 arithmetic restated in C, one thread) on a bounded sample of the same

@@ -57,7 +57,7 @@ struct ReduceOut {
     double nonfinite;
 };
 
-// The chunks of 2 x 8 tiles that one launch of the fused step works on,
+// The chunks of 2 x 4 tiles that one launch of the fused step works on,
 // when not all: n chunk indices (row-major chunk order, c = cy * chunks_x +
 // cx) in the order the launch deals them to the XCDs.  Every chunk left out
 // holds phi = +0 and T = +0 on both buffers and keeps them over the launch,

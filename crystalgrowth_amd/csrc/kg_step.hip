@@ -225,17 +225,19 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
 // Order of the tiles over the blocks of the single-grid kernel.  Block b
 // runs on XCD b % 8 (workgroups go round-robin over the XCDs) and takes the
 // (b / 8)-th tile of that XCD's sequence.  The grid is cut into chunks of
-// 2 x 8 tiles; chunks go round-robin over the XCDs in row-major chunk order,
+// 2 x 4 tiles; chunks go round-robin over the XCDs in row-major chunk order,
 // each chunk row rotated by 3 chunks, and an XCD walks its chunks one after
-// the other, the 16 tiles of a chunk in row-major order.  So at any time
+// the other, the 8 tiles of a chunk in row-major order.  So at any time
 // every XCD works on tiles spread over the grid (clustered dendrites load
 // the XCDs evenly, and cheap and expensive tiles run side by side), while
 // tiles that share halos are processed together on one XCD and meet in its
 // L2.  Measured on 20000 steps of a 4096 x 4096 grid growing nine seeds
-// (mean of 10 checkpoint steps): 55.1 us row-major, 55.8 us one band per
-// XCD, 46.0 us 2 x 8 chunks unrotated, 44.3 us rotated; permuting the chunk
-// order (49.7 - 51.5 us) loses the reuse between neighbouring chunks.
-constexpr int CHUNK_W = 2, CHUNK_H = 8, CHUNK_SKEW = 3;
+// (mean of 10 checkpoint steps), all chunks launched: 55.1 us row-major,
+// 55.8 us one band per XCD, 46.0 us 2 x 8 chunks unrotated, 44.3 us rotated
+// (2 x 4 the same); permuting the chunk order (49.7 - 51.5 us) loses the
+// reuse between neighbouring chunks.  With launches of the live chunks only
+// (below) the smaller chunk skips more: 40.7 us at 2 x 8, 40.1 us at 2 x 4.
+constexpr int CHUNK_W = 2, CHUNK_H = 4, CHUNK_SKEW = 3;
 
 // Tile (tx, ty) of this block and its chunk's row-major index c; false for
 // blocks past the grid.

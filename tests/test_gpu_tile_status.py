@@ -129,7 +129,7 @@ def test_chunk_skipping_matches_dense_bitwise(kb, boundary, prec):
     the nucleus sits at a corner, so its fields wrap to all four edges.
     Steps are taken in uneven calls, so the host's request periods fall
     across call boundaries."""
-    nx, ny = 1024, 960         # 16 x 32 tiles, 8 x 4 chunks
+    nx, ny = 1024, 960         # 16 x 32 tiles, 8 x 8 chunks of 2 x 4 tiles
     p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
                           boundary=boundary, seed=4)
     cx, cy = (3, 2) if boundary == 0 else (150, 140)
