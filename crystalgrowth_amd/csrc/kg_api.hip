@@ -111,7 +111,8 @@ int ensure_status(kg_solver_s &h)
 // and (2) asks again: k_chunk_live reads the
 // status as the stream stands and writes its flags to pinned host memory,
 // behind an event.  A chunk is skipped when every tile within
-// SKIP_MARGIN_TILES tiles of it is exactly phi = +0, T = +0 on both buffers
+// SKIP_MARGIN_X tile columns and SKIP_MARGIN_Y tile rows of it (64 and 60
+// cells) is exactly phi = +0, T = +0 on both buffers
 // with status words saying so; nonzero values cannot reach the chunk in the
 // 3 * SKIP_PERIOD steps an answer is used for (kg_step.hip checks the
 // margin), and a step of such a tile would write the same zeros and status

@@ -81,9 +81,10 @@ int fused_chunks_x(int nx);
 int fused_chunks_y(int ny);
 // Tile rows/columns of margin around a chunk that launch_chunk_live checks,
 // and the period (steps) of the host's liveness requests.
-constexpr int SKIP_MARGIN_TILES = 2;
+constexpr int SKIP_MARGIN_X = 1, SKIP_MARGIN_Y = 2;  // tile columns, tile rows
 constexpr int SKIP_PERIOD = 8;
-// live[c] = 1 if some tile within SKIP_MARGIN_TILES tiles of chunk c (mapped
+// live[c] = 1 if some tile within SKIP_MARGIN_X tile columns and
+// SKIP_MARGIN_Y tile rows of chunk c (mapped
 // by the boundary) has a status word other than exactly "phi = +0, T = +0"
 // in either set (stat_a, stat_b); 0 otherwise.
 hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,

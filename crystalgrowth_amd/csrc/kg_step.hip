@@ -937,14 +937,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 // (pass 2 reads pass-1 values of neighbours, which read phi one cell
 // further), and a tile reads 2 cells beyond itself; the margin must cover
 // both.
-static_assert(2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_TILES * TY &&
-                  2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_TILES * TX,
+static_assert(2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_Y * TY &&
+                  2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_X * TX,
               "chunk skip margin too small for the request period");
 
 // Liveness of the chunks (see launch_chunk_live): one block per chunk, one
 // thread per tile of the chunk and its margin, all status loads in flight
 // together.
-constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_TILES, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_TILES;
+constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_X, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_Y;
 constexpr int LIVE_THREADS = (LIVE_W * LIVE_H + 63) / 64 * 64;
 __global__ __launch_bounds__(LIVE_THREADS) void k_chunk_live(const uint32_t *__restrict__ sa,
                                                              const uint32_t *__restrict__ sb,
@@ -959,8 +959,8 @@ __global__ __launch_bounds__(LIVE_THREADS) void k_chunk_live(const uint32_t *__r
     int lv = 0;
     const int k = threadIdx.x;
     if (k < LIVE_W * LIVE_H) {
-        const int y = cy * CHUNK_H - SKIP_MARGIN_TILES + k / LIVE_W;
-        const int x = cx * CHUNK_W - SKIP_MARGIN_TILES + k % LIVE_W;
+        const int y = cy * CHUNK_H - SKIP_MARGIN_Y + k / LIVE_W;
+        const int x = cx * CHUNK_W - SKIP_MARGIN_X + k % LIVE_W;
         int yy, xx;
         if (boundary == 1) {
             yy = y < 0 ? 0 : (y >= tiles_y ? tiles_y - 1 : y);
