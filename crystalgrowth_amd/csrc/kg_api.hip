@@ -100,44 +100,51 @@ int ensure_status(kg_solver_s &h)
     h.stat_valid = true;
     // liveness answers about the old fields are void
     h.live.n = -1;
-    h.live_pending[0] = h.live_pending[1] = false;
+    for (bool &q : h.live_pending)
+        q = false;
     return KG_OK;
 }
 
 // Chunks the fused step works on.  At every step that is a multiple of
-// SKIP_PERIOD the host (1) takes the liveness answer it asked for two
+// SKIP_PERIOD the host (1) takes the liveness answer it asked for SKIP_AGE
 // periods ago, if any, as the list of chunks the next SKIP_PERIOD launches
-// cover (no answer, or more live chunks than a LiveList holds: all chunks),
-// and (2) asks again: k_chunk_live reads the
-// status as the stream stands and writes its flags to pinned host memory,
-// behind an event.  A chunk is skipped when every tile within
-// SKIP_MARGIN_X tile columns and SKIP_MARGIN_Y tile rows of it (64 and 60
-// cells) is exactly phi = +0, T = +0 on both buffers
-// with status words saying so; nonzero values cannot reach the chunk in the
-// 3 * SKIP_PERIOD steps an answer is used for (kg_step.hip checks the
-// margin), and a step of such a tile would write the same zeros and status
-// the buffers hold already, so skipping changes no bit.  Waiting on an
-// answer bounds how far the host runs ahead of the GPU to about two periods.
+// cover (no answer, an answer not all written yet, or more live chunks than
+// a LiveList holds: all chunks), and (2) asks again: k_chunk_live reads the
+// status as the stream stands and writes its flags, tagged with the
+// request's number, to pinned host memory, behind an event.  A chunk is
+// skipped when every tile within SKIP_MARGIN_X tile columns and
+// SKIP_MARGIN_Y tile rows of it (64 and 30 cells) is exactly phi = +0, T =
+// +0 on both buffers with status words saying so; nonzero values cannot
+// reach the chunk in the (SKIP_AGE + 1) * SKIP_PERIOD steps an answer is
+// used for (kg_step.hip checks the margin), and a step of such a tile would
+// write the same zeros and status the buffers hold already, so skipping
+// changes no bit.  The event skips the system-scope fence (a write-back of
+// the L2 that would cost microseconds): the flags do not need it, being
+// written with system-scope stores, and a flag not yet visible shows by its
+// tag.  Waiting on an answer bounds how far the host runs ahead of the GPU
+// to about SKIP_AGE periods.
 int skip_cycle(kg_solver_s &h)
 {
-    const int p = (int)((h.steps / kg::SKIP_PERIOD) & 1);
+    const int p = (int)((h.steps / kg::SKIP_PERIOD) % kg::SKIP_AGE);
     h.live.n = -1;
     if (h.live_pending[p]) {
         KG_HIP(hipEventSynchronize(h.live_ev[p]));
         h.live_order.clear();
-        kg::live_chunk_order(h.nx, h.ny, h.live_host[p], h.live_order);
-        if (h.live_order.size() <= (size_t)kg::LIVE_MAX) {
+        if (!kg::live_chunk_order(h.nx, h.ny, h.live_host[p], h.live_tag[p], h.live_order))
+            ++h.live_stale;
+        else if (h.live_order.size() <= (size_t)kg::LIVE_MAX) {
             std::copy(h.live_order.begin(), h.live_order.end(), h.live.c);
             h.live.n = (int)h.live_order.size();
         }
         h.live_pending[p] = false;
     }
-    // the kernel writes the flags straight into the pinned host buffer (a
-    // copy on the stream would hold the next step back)
+    h.next_tag = h.next_tag % kg::LIVE_TAGS + 1;
     KG_HIP(kg::launch_chunk_live(h.stat + (size_t)h.cur * h.stat_tiles,
                                  h.stat + (size_t)(h.cur ^ 1) * h.stat_tiles, h.tiles_x,
-                                 h.tiles_y, h.params.boundary, h.live_host[p], h.stream));
+                                 h.tiles_y, h.params.boundary, h.next_tag, h.live_host[p],
+                                 h.stream));
     KG_HIP(hipEventRecord(h.live_ev[p], h.stream));
+    h.live_tag[p] = h.next_tag;
     h.live_pending[p] = true;
     return KG_OK;
 }
@@ -161,7 +168,7 @@ int step_impl(kg_solver_s &h, int nsteps)
         invalidate_status(h);  // the step writes the buffers without keeping the status
     else if (int rc = ensure_status(h))
         return rc;
-    const bool skipping = h.kernel == KG_KERNEL_FUSED && h.stat_ok && h.live_host[1] != nullptr;
+    const bool skipping = h.kernel == KG_KERNEL_FUSED && h.stat_ok && h.live_host[0] != nullptr;
     for (int s = 0; s < nsteps; ++s) {
         if (skipping && h.steps % kg::SKIP_PERIOD == 0)
             if (int rc = skip_cycle(h))
@@ -233,7 +240,7 @@ void free_all(kg_solver_s *h)
     for (void *b : bufs)
         if (b)
             (void)hipFree(b);
-    for (int p = 0; p < 2; ++p) {
+    for (int p = 0; p < kg::SKIP_AGE; ++p) {
         if (h->live_host[p])
             (void)hipHostFree(h->live_host[p]);
         if (h->live_ev[p])
@@ -421,13 +428,16 @@ int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out)
         if (nc <= 65535) {
             h->nchunks = nc;
             bool ok = true;
-            for (int q = 0; q < 2 && ok; ++q) {
+            for (int q = 0; q < kg::SKIP_AGE && ok; ++q) {
                 ok = hipHostMalloc(reinterpret_cast<void **>(&h->live_host[q]), (size_t)nc,
-                                   hipHostMallocDefault) == hipSuccess;
+                                   hipHostMallocCoherent) == hipSuccess;
                 if (!ok)
                     h->live_host[q] = nullptr;
-                ok = ok && hipEventCreateWithFlags(&h->live_ev[q], hipEventDisableTiming) ==
-                               hipSuccess;
+                else
+                    std::memset(h->live_host[q], 0, (size_t)nc);  // tag 0: never written
+                ok = ok && hipEventCreateWithFlags(&h->live_ev[q],
+                                                   hipEventDisableTiming |
+                                                       hipEventDisableSystemFence) == hipSuccess;
             }
             if (!ok) {
                 free_all(h);

@@ -41,15 +41,18 @@ struct kg_solver_s {
     int stat_tiles = 0, tiles_x = 0, tiles_y = 0;
     bool stat_ok = false, stat_valid = false;
     // chunks the fused step works on (kg_api.hip, skip_cycle): the list in
-    // use (n < 0: all), and two pinned host buffers of liveness flags, each
-    // with the event of the kernel that writes it and whether it is still to
-    // be read
+    // use (n < 0: all), and SKIP_AGE pinned host buffers of liveness flags,
+    // each with the event of the kernel that writes it, the tag that request
+    // writes and whether it is still to be read
     kg::LiveList live{-1, {}};
     std::vector<uint16_t> live_order;
     int nchunks = 0;
-    uint8_t *live_host[2] = {nullptr, nullptr};
-    hipEvent_t live_ev[2] = {nullptr, nullptr};
-    bool live_pending[2] = {false, false};
+    uint8_t *live_host[kg::SKIP_AGE] = {};
+    hipEvent_t live_ev[kg::SKIP_AGE] = {};
+    uint32_t live_tag[kg::SKIP_AGE] = {};
+    bool live_pending[kg::SKIP_AGE] = {};
+    uint32_t next_tag = 0;
+    uint64_t live_stale = 0;  // answers found not yet written (full launches instead)
     // bound on |theta| of angles not written by the step (resets write 0,
     // kg_set_fields what it is given; infinite after kg_device_fields)
     double theta_given = 0.0;

@@ -73,22 +73,30 @@ static_assert(LIVE_MAX % 2 == 0, "LiveList entries are read in pairs");
 // live == nullptr or live->n < 0: every chunk
 template <typename R>
 hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live = nullptr);
-// Append to out the chunks flagged in flags[c] (one byte per chunk), in the
-// order a LiveList must hold them.
-void live_chunk_order(int nx, int ny, const uint8_t *flags, std::vector<uint16_t> &out);
+// Append to out the chunks flagged live in flags[c] (one byte per chunk, as
+// launch_chunk_live writes it), in the order a LiveList must hold them.
+// False, with out incomplete, if some byte does not carry `tag` (not yet
+// written by the request that tag names).
+bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
+                      std::vector<uint16_t> &out);
 // Chunk grid of the fused step for an nx x ny grid (chunks_x * chunks_y chunks).
 int fused_chunks_x(int nx);
 int fused_chunks_y(int ny);
 // Tile rows/columns of margin around a chunk that launch_chunk_live checks,
 // and the period (steps) of the host's liveness requests.
-constexpr int SKIP_MARGIN_X = 1, SKIP_MARGIN_Y = 2;  // tile columns, tile rows
-constexpr int SKIP_PERIOD = 8;
-// live[c] = 1 if some tile within SKIP_MARGIN_X tile columns and
-// SKIP_MARGIN_Y tile rows of chunk c (mapped
-// by the boundary) has a status word other than exactly "phi = +0, T = +0"
-// in either set (stat_a, stat_b); 0 otherwise.
+constexpr int SKIP_MARGIN_X = 1, SKIP_MARGIN_Y = 1;  // tile columns, tile rows
+constexpr int SKIP_PERIOD = 7;
+// periods between a liveness request and the first launch that uses it
+constexpr int SKIP_AGE = 1;
+// live[c] = tag << 1 | 1 if some tile within SKIP_MARGIN_X tile columns and
+// SKIP_MARGIN_Y tile rows of chunk c (mapped by the boundary) has a status
+// word other than exactly "phi = +0, T = +0" in either set (stat_a,
+// stat_b), tag << 1 otherwise; tag in 1..LIVE_TAGS.  live is host memory;
+// each byte is written with a system-scope store.
+constexpr uint32_t LIVE_TAGS = 127;
 hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,
-                             int tiles_y, int boundary, uint8_t *live, hipStream_t s);
+                             int tiles_y, int boundary, uint32_t tag, uint8_t *live,
+                             hipStream_t s);
 template <typename R>
 hipError_t launch_step_two_pass(const StepArgs<R> &a, const PassTemps<R> &t, hipStream_t s);
 

@@ -931,50 +931,59 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 }
 
 // A chunk skipped by a launch must stay all +0 for the launch.  The host uses
-// a liveness answer for the SKIP_PERIOD steps that start two periods after
-// it asked, so up to 3 SKIP_PERIOD steps separate the status it saw from the
-// last step that uses it.  A step moves nonzero values at most 2 cells
+// a liveness answer for the SKIP_PERIOD steps that start SKIP_AGE periods
+// after it asked, so up to (SKIP_AGE + 1) SKIP_PERIOD steps separate the
+// status it saw from the last step that uses it.  A step moves nonzero values at most 2 cells
 // (pass 2 reads pass-1 values of neighbours, which read phi one cell
 // further), and a tile reads 2 cells beyond itself; the margin must cover
 // both.
-static_assert(2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_Y * TY &&
-                  2 * 3 * SKIP_PERIOD + 2 <= SKIP_MARGIN_X * TX,
+static_assert(2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_Y * TY &&
+                  2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_X * TX,
               "chunk skip margin too small for the request period");
 
-// Liveness of the chunks (see launch_chunk_live): one block per chunk, one
-// thread per tile of the chunk and its margin, all status loads in flight
-// together.
+// Liveness of the chunks (see launch_chunk_live): one thread per chunk, its
+// tile rows and columns (margin included) mapped onto the grid first, then
+// every status load in flight together.  Flag byte = tag << 1 | live, stored
+// with a system-scope store so that it goes straight to the host buffer (the
+// host checks the tag, so it needs no fence that writes back the L2).
 constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_X, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_Y;
-constexpr int LIVE_THREADS = (LIVE_W * LIVE_H + 63) / 64 * 64;
+constexpr int LIVE_THREADS = 64;
+__device__ __forceinline__ int live_wrap(int v, int n, int boundary)
+{
+    if (boundary == 1)
+        return v < 0 ? 0 : (v >= n ? n - 1 : v);
+    return ((v % n) + n) % n;
+}
 __global__ __launch_bounds__(LIVE_THREADS) void k_chunk_live(const uint32_t *__restrict__ sa,
                                                              const uint32_t *__restrict__ sb,
                                                              int tiles_x, int tiles_y,
-                                                             int boundary,
+                                                             int boundary, int nchunks,
+                                                             uint32_t tag,
                                                              uint8_t *__restrict__ live)
 {
     const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
-    const int c = blockIdx.x;
+    const int c = blockIdx.x * LIVE_THREADS + threadIdx.x;
+    if (c >= nchunks)
+        return;
     const int cx = c % ncx, cy = c / ncx;
     constexpr uint32_t Q = S_PHI0 | S_T0;
-    int lv = 0;
-    const int k = threadIdx.x;
-    if (k < LIVE_W * LIVE_H) {
-        const int y = cy * CHUNK_H - SKIP_MARGIN_Y + k / LIVE_W;
-        const int x = cx * CHUNK_W - SKIP_MARGIN_X + k % LIVE_W;
-        int yy, xx;
-        if (boundary == 1) {
-            yy = y < 0 ? 0 : (y >= tiles_y ? tiles_y - 1 : y);
-            xx = x < 0 ? 0 : (x >= tiles_x ? tiles_x - 1 : x);
-        } else {
-            yy = ((y % tiles_y) + tiles_y) % tiles_y;
-            xx = ((x % tiles_x) + tiles_x) % tiles_x;
+    int col[LIVE_W], row[LIVE_H];
+#pragma unroll
+    for (int i = 0; i < LIVE_W; ++i)
+        col[i] = live_wrap(cx * CHUNK_W - SKIP_MARGIN_X + i, tiles_x, boundary);
+#pragma unroll
+    for (int j = 0; j < LIVE_H; ++j)
+        row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y + j, tiles_y, boundary) * tiles_x;
+    uint32_t any = 0;
+#pragma unroll
+    for (int j = 0; j < LIVE_H; ++j)
+#pragma unroll
+        for (int i = 0; i < LIVE_W; ++i) {
+            const int t = row[j] + col[i];
+            any |= (sa[t] ^ Q) | (sb[t] ^ Q);
         }
-        const int t = yy * tiles_x + xx;
-        lv = (sa[t] != Q) | (sb[t] != Q);
-    }
-    lv = __syncthreads_or(lv);
-    if (k == 0)
-        live[c] = lv ? 1 : 0;
+    __hip_atomic_store(live + c, (uint8_t)(tag << 1 | (any ? 1u : 0u)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Batched step: every tile of every ensemble member in one launch.  tiles[t]
@@ -1305,7 +1314,8 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList
 
 // The chunk indices of the chunks flagged live, in the order chunk_tile
 // deals all chunks (chunk rows rotated by CHUNK_SKEW): appended to out.
-void live_chunk_order(int nx, int ny, const uint8_t *flags, std::vector<uint16_t> &out)
+bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
+                      std::vector<uint16_t> &out)
 {
     const int tiles_x = (nx + TX - 1) / TX, tiles_y = (ny + TY - 1) / TY;
     const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
@@ -1313,20 +1323,26 @@ void live_chunk_order(int nx, int ny, const uint8_t *flags, std::vector<uint16_t
         const int cy = chunk / ncx;
         const int cx = (chunk - cy * ncx + CHUNK_SKEW * cy) % ncx;
         const int c = cy * ncx + cx;
-        if (flags[c])
+        const uint8_t f = flags[c];
+        if ((uint32_t)(f >> 1) != tag)
+            return false;
+        if (f & 1)
             out.push_back((uint16_t)c);
     }
+    return true;
 }
 
 int fused_chunks_x(int nx) { return ((nx + TX - 1) / TX + CHUNK_W - 1) / CHUNK_W; }
 int fused_chunks_y(int ny) { return ((ny + TY - 1) / TY + CHUNK_H - 1) / CHUNK_H; }
 
 hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,
-                             int tiles_y, int boundary, uint8_t *live, hipStream_t s)
+                             int tiles_y, int boundary, uint32_t tag, uint8_t *live,
+                             hipStream_t s)
 {
     const int n = ((tiles_x + CHUNK_W - 1) / CHUNK_W) * ((tiles_y + CHUNK_H - 1) / CHUNK_H);
-    hipLaunchKernelGGL(k_chunk_live, dim3(n), dim3(LIVE_THREADS), 0, s, stat_a, stat_b, tiles_x,
-                       tiles_y, boundary, live);
+    hipLaunchKernelGGL(k_chunk_live, dim3((n + LIVE_THREADS - 1) / LIVE_THREADS),
+                       dim3(LIVE_THREADS), 0, s, stat_a, stat_b, tiles_x, tiles_y, boundary, n,
+                       tag, live);
     return hipGetLastError();
 }
 

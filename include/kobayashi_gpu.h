@@ -124,8 +124,9 @@ int kg_get_size(kg_handle h, int *nx, int *ny, int *precision);
 /* nsteps explicit-Euler steps: _computeGradientLaplacian + _evolution
  * (src/Kobayashi.cpp:125-221) per step; Kobayashi::iUpdate is kg_step(h, 10)
  * (src/Kobayashi.cpp:227-239).  Asynchronous on the handle's stream; with
- * the default kernel the host stays at most about 16 steps ahead of the GPU
- * (it waits for the GPU's answer on which regions stay all +0). */
+ * the default kernel the host stays at most about 14 steps ahead of the GPU
+ * (every 7 steps it waits for the GPU's answer on which regions stay all
+ * +0). */
 int kg_step(kg_handle h, int nsteps);
 /* kg_step bracketed by HIP events on the handle's stream; *ms = elapsed
  * milliseconds of the nsteps launches (waits for completion). */
