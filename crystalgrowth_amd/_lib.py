@@ -110,6 +110,7 @@ SIGNATURES = {
     "kg_step_timed": [H, I, ctypes.POINTER(ctypes.c_double)],
     "kg_synchronize": [H],
     "kg_get_step_count": [H, ctypes.POINTER(U64)],
+    "kg_get_skip_stats": [H, ctypes.POINTER(U64), ctypes.POINTER(U64)],
     "kg_set_step_count": [H, U64],
     "kg_set_kernel": [H, I],
     "kg_get_phase": [H, VP, SZ],

@@ -184,6 +184,8 @@ int step_impl(kg_solver_s &h, int nsteps)
             e = kg::launch_step_two_pass<R>(a, t, h.stream);
         } else {
             e = kg::launch_step_fused<R>(a, h.stream, skipping ? &h.live : nullptr);
+            if (skipping && h.live.n >= 0 && h.live.n < h.nchunks)
+                ++h.listed_launches;
         }
         if (e != hipSuccess)
             return hip_fail(e, "step launch");
@@ -423,9 +425,11 @@ int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out)
             free_all(h);
             return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the tile status failed");
         }
-        // chunk lists: chunk indices must fit 16 bits
+        // chunk lists: chunk indices must fit 16 bits; smaller grids do not
+        // fill the GPU and gain less from skipping than the host's waits
+        // cost when handles are stepped in turn
         const int nc = kg::fused_chunks_x(nx) * kg::fused_chunks_y(ny);
-        if (nc <= 65535) {
+        if (nc >= kg::SKIP_MIN_CHUNKS && nc <= 65535) {
             h->nchunks = nc;
             bool ok = true;
             for (int q = 0; q < kg::SKIP_AGE && ok; ++q) {
@@ -577,6 +581,17 @@ int kg_get_step_count(kg_handle h, uint64_t *steps)
     if (!steps)
         return fail(KG_ERR_INVALID_ARGUMENT, "null steps");
     *steps = h->steps;
+    return KG_OK;
+}
+
+int kg_get_skip_stats(kg_handle h, uint64_t *listed_launches, uint64_t *stale_answers)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!listed_launches || !stale_answers)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null output");
+    *listed_launches = h->listed_launches;
+    *stale_answers = h->live_stale;
     return KG_OK;
 }
 

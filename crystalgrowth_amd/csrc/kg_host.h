@@ -52,7 +52,8 @@ struct kg_solver_s {
     uint32_t live_tag[kg::SKIP_AGE] = {};
     bool live_pending[kg::SKIP_AGE] = {};
     uint32_t next_tag = 0;
-    uint64_t live_stale = 0;  // answers found not yet written (full launches instead)
+    uint64_t live_stale = 0;       // answers found not yet written (full launches instead)
+    uint64_t listed_launches = 0;  // step launches that covered only listed chunks
     // bound on |theta| of angles not written by the step (resets write 0,
     // kg_set_fields what it is given; infinite after kg_device_fields)
     double theta_given = 0.0;

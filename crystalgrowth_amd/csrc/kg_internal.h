@@ -88,6 +88,11 @@ constexpr int SKIP_MARGIN_X = 1, SKIP_MARGIN_Y = 1;  // tile columns, tile rows
 constexpr int SKIP_PERIOD = 7;
 // periods between a liveness request and the first launch that uses it
 constexpr int SKIP_AGE = 1;
+// grids with fewer chunks launch every chunk: a chunk list saves them at
+// most about a microsecond per step, while its waits keep the host within
+// two periods of the GPU, so handles stepped in turn would stop overlapping
+// on the GPU (256 chunks of 2 x 4 tiles: about 2000 x 2000 cells)
+constexpr int SKIP_MIN_CHUNKS = 256;
 // live[c] = tag << 1 | 1 if some tile within SKIP_MARGIN_X tile columns and
 // SKIP_MARGIN_Y tile rows of chunk c (mapped by the boundary) has a status
 // word other than exactly "phi = +0, T = +0" in either set (stat_a,

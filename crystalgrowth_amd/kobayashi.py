@@ -133,6 +133,13 @@ class Solver:
     def step_count(self, v: int) -> None:
         check(lib().kg_set_step_count(self._h, int(v)))
 
+    def skip_stats(self) -> dict:
+        """Launches that covered only the live chunks, and liveness answers
+        dropped as not yet visible (kg_get_skip_stats)."""
+        listed, stale = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        check(lib().kg_get_skip_stats(self._h, ctypes.byref(listed), ctypes.byref(stale)))
+        return {"listed_launches": listed.value, "stale_answers": stale.value}
+
     # -- fields -----------------------------------------------------------
     def _get(self, fn) -> np.ndarray:
         a = np.empty((self.ny, self.nx), self.dtype)

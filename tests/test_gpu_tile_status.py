@@ -22,8 +22,9 @@ def kb():
     return kb
 
 
-def run_pair(kb, nx, ny, prec, p, setup, marks):
-    """Advance a status solver and a dense solver side by side; fields at each mark."""
+def run_pair(kb, nx, ny, prec, p, setup, marks, stats=None):
+    """Advance a status solver and a dense solver side by side; fields at each
+    mark (and the status solver's skip statistics appended to stats)."""
     out = []
     with kb.Solver(nx, ny, prec, p) as a, kb.Solver(nx, ny, prec, p) as b:
         b.set_kernel("fused_dense")
@@ -35,6 +36,8 @@ def run_pair(kb, nx, ny, prec, p, setup, marks):
             b.step(m - done)
             done = m
             out.append((m, a.fields(), b.fields()))
+        if stats is not None:
+            stats.append(a.skip_stats())
     return out
 
 
@@ -129,19 +132,31 @@ def test_chunk_skipping_matches_dense_bitwise(kb, boundary, prec):
     the nucleus sits at a corner, so its fields wrap to all four edges.
     Steps are taken in uneven calls, so the host's request periods fall
     across call boundaries."""
-    nx, ny = 1024, 960         # 16 x 32 tiles, 8 x 8 chunks of 2 x 4 tiles
+    nx, ny = 2048, 1920        # 32 x 64 tiles, 16 x 16 chunks of 2 x 4 tiles
     p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
                           boundary=boundary, seed=4)
     cx, cy = (3, 2) if boundary == 0 else (150, 140)
+    stats = []
     pairs = run_pair(kb, nx, ny, prec, p, lambda s: s.reset_disk(cx, cy, 6.0),
-                     [1, 6, 13, 31, 90, 301, 700])
+                     [1, 6, 13, 31, 90, 301, 700], stats)
     assert_bitwise(pairs)
+    # most launches after the first answer cover only the live chunks
+    assert stats[0]["listed_launches"] >= 600 and stats[0]["stale_answers"] == 0, stats
+
+
+def test_small_grid_launches_every_chunk(kb):
+    """Below 256 chunks the handle never waits for liveness answers."""
+    with kb.Solver(1024, 960, "f32", kb.default_params(noise_a=0.01, seed=2)) as s:
+        s.reset_disk(100, 100, 5.0)
+        s.step(100)
+        s.synchronize()
+        assert s.skip_stats() == {"listed_launches": 0, "stale_answers": 0}
 
 
 def test_chunk_skipping_after_new_fields(kb):
     """A blob written far from the old activity while skip answers about the
     old fields are still in flight: the answers must be dropped."""
-    nx, ny = 1024, 960
+    nx, ny = 2048, 1920
     p = kb.default_params(noise_a=0.01, boundary=1, seed=6)
     with kb.Solver(nx, ny, "f32", p) as a, kb.Solver(nx, ny, "f32", p) as b:
         b.set_kernel("fused_dense")
@@ -149,9 +164,9 @@ def test_chunk_skipping_after_new_fields(kb):
             s.reset_disk(100, 100, 5.0)
             s.step(41)
         phi = np.zeros((ny, nx), np.float32)
-        phi[800:812, 900:915] = 1.0
+        phi[1700:1712, 1800:1815] = 1.0
         T = np.zeros((ny, nx), np.float32)
-        T[700:720, 20:60] = 0.2
+        T[1500:1520, 20:60] = 0.2
         for s in (a, b):
             s.set_fields(phi, T)
             s.step(3)
