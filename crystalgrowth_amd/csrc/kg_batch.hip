@@ -70,7 +70,7 @@ void free_batch(kg_batch_s *b)
 }
 
 // The tile table, in block order: block b steps the tile of entry b.  Every
-// member is cut into chunks of 2 x 8 tiles (chunk rows rotated by 3, as the
+// member is cut into chunks of 2 x 4 tiles (chunk rows rotated by 3, as the
 // single-grid kernel orders its tiles); the chunks, member after member, are
 // dealt round-robin to the 8 XCDs (block b runs on XCD b % 8), each XCD
 // walks its chunks one after the other, and its sequence is padded to the
@@ -79,7 +79,7 @@ void free_batch(kg_batch_s *b)
 // *real = number of tiles (entries that are not padding).
 std::vector<int4> batch_tile_table(const int *nx, const int *ny, int count, size_t *real)
 {
-    constexpr int NX = 8, CW = 2, CH = 8, SKEW = 3;
+    constexpr int NX = 8, CW = 2, CH = 4, SKEW = 3;
     const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
     std::vector<std::vector<int4>> seq(NX);
     size_t chunk = 0, n = 0;
