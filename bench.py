@@ -67,6 +67,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=5000,
                     help="steps of the batched-vs-looped comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-looped", action="store_true",
+                    help="batched leg without the looped plain handles (for profiling)")
     ap.add_argument("--no-dense", action="store_true",
                     help="skip the same trajectory on the fused kernel without tile status")
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -226,7 +228,7 @@ def calibrate_copy():
                       "bytes_written_per_copy": 4 * n, "copies": 20}), flush=True)
 
 
-def batched_vs_looped(kb, nsteps: int):
+def batched_vs_looped(kb, nsteps: int, looped: bool = True):
     """The ensemble from its initial state (disk r = 5 at every member's
     centre) for nsteps steps: one batched launch per step vs each member's
     own plain handle stepped in turn (every handle has its own stream, so
@@ -245,6 +247,10 @@ def batched_vs_looped(kb, nsteps: int):
         b.synchronize()
         tb = time.perf_counter() - t0
         sb = [b.stats(m)["solid_fraction"] for m in (0, len(sizes) // 2, len(sizes) - 1)]
+    if not looped:
+        return {"members": len(sizes), "cells": cells, "steps": nsteps,
+                "batched_Gcell_updates_per_s": round(cells * nsteps / tb / 1e9, 3),
+                "launch_mode": [mode, g], "batched_s": round(tb, 3)}
     solvers = [kb.Solver(nx, ny, "f32", p) for (nx, ny), p in zip(sizes, params)]
     for (nx, ny), s in zip(sizes, solvers):
         s.reset_disk(nx // 2, ny // 2, 5.0)
@@ -376,7 +382,7 @@ def main():
     if not args.no_fp64_check:
         rec["fp64_rel_err_100_steps"] = fp64_error(kb, n)
     if args.batch_steps > 0:
-        rec["batched"] = batched_vs_looped(kb, args.batch_steps)
+        rec["batched"] = batched_vs_looped(kb, args.batch_steps, not args.no_looped)
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_vs_forward(kb)
     rec["cpu_baseline"] = cpu_baseline(n, args.cpu_steps) if args.cpu_steps > 0 else None
