@@ -88,6 +88,8 @@ def test_null_handle_errors(L):
     assert L.kg_step(None, 1) == _lib.KG_ERR_INVALID_ARGUMENT
     assert L.kg_destroy(None) == _lib.KG_ERR_INVALID_ARGUMENT
     assert b"null" in L.kg_last_error()
+    u, v = ctypes.c_uint64(), ctypes.c_uint64()
+    assert L.kg_get_skip_stats(None, ctypes.byref(u), ctypes.byref(v)) == _lib.KG_ERR_INVALID_ARGUMENT
 
 
 def test_create_without_gpu_fails_loudly(L):
