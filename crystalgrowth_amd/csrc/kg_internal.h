@@ -74,9 +74,11 @@ static_assert(LIVE_MAX % 2 == 0, "LiveList entries are read in pairs");
 template <typename R>
 hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live = nullptr);
 // Append to out the chunks flagged live in flags[c] (one byte per chunk, as
-// launch_chunk_live writes it), in the order a LiveList must hold them.
-// False, with out incomplete, if some byte does not carry `tag` (not yet
-// written by the request that tag names).
+// launch_chunk_live writes it), in the order a LiveList must hold them: by
+// their count of active tiles, most first (a launch then ends on cheap
+// tiles), the full launch's chunk order within one count.  False, with out
+// incomplete, if some byte does not carry `tag` (not yet written by the
+// request that tag names).
 bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
                       std::vector<uint16_t> &out);
 // Chunk grid of the fused step for an nx x ny grid (chunks_x * chunks_y chunks).
@@ -93,12 +95,13 @@ constexpr int SKIP_AGE = 1;
 // two periods of the GPU, so handles stepped in turn would stop overlapping
 // on the GPU (256 chunks of 2 x 4 tiles: about 2000 x 2000 cells)
 constexpr int SKIP_MIN_CHUNKS = 256;
-// live[c] = tag << 1 | 1 if some tile within SKIP_MARGIN_X tile columns and
-// SKIP_MARGIN_Y tile rows of chunk c (mapped by the boundary) has a status
-// word other than exactly "phi = +0, T = +0" in either set (stat_a,
-// stat_b), tag << 1 otherwise; tag in 1..LIVE_TAGS.  live is host memory;
-// each byte is written with a system-scope store.
-constexpr uint32_t LIVE_TAGS = 127;
+// live[c] = tag << 5 | na << 1 | lv: lv = 1 if some tile within
+// SKIP_MARGIN_X tile columns and SKIP_MARGIN_Y tile rows of chunk c (mapped
+// by the boundary) has a status word other than exactly "phi = +0, T = +0"
+// in either set (stat_a, stat_b); na = tiles of chunk c whose stat_a word
+// says the last step took the full path; tag in 1..LIVE_TAGS.  live is host
+// memory; each byte is written with a system-scope store.
+constexpr uint32_t LIVE_TAGS = 7;
 hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,
                              int tiles_y, int boundary, uint32_t tag, uint8_t *live,
                              hipStream_t s);

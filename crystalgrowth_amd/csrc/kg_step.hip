@@ -937,15 +937,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 // (pass 2 reads pass-1 values of neighbours, which read phi one cell
 // further), and a tile reads 2 cells beyond itself; the margin must cover
 // both.
+static_assert(CHUNK_W * CHUNK_H < 16, "active-tile count fits 4 bits");
 static_assert(2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_Y * TY &&
                   2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_X * TX,
               "chunk skip margin too small for the request period");
 
 // Liveness of the chunks (see launch_chunk_live): one thread per chunk, its
 // tile rows and columns (margin included) mapped onto the grid first, then
-// every status load in flight together.  Flag byte = tag << 1 | live, stored
-// with a system-scope store so that it goes straight to the host buffer (the
-// host checks the tag, so it needs no fence that writes back the L2).
+// every status load in flight together.  Flag byte = tag << 5 | active
+// tiles << 1 | live, stored with a system-scope store so that it goes
+// straight to the host buffer (the host checks the tag, so it needs no fence
+// that writes back the L2).
 constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_X, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_Y;
 constexpr int LIVE_THREADS = 64;
 __device__ __forceinline__ int live_wrap(int v, int n, int boundary)
@@ -974,16 +976,20 @@ __global__ __launch_bounds__(LIVE_THREADS) void k_chunk_live(const uint32_t *__r
 #pragma unroll
     for (int j = 0; j < LIVE_H; ++j)
         row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y + j, tiles_y, boundary) * tiles_x;
-    uint32_t any = 0;
+    uint32_t any = 0, act = 0;  // act: tiles of the chunk that took the full path
 #pragma unroll
     for (int j = 0; j < LIVE_H; ++j)
 #pragma unroll
         for (int i = 0; i < LIVE_W; ++i) {
             const int t = row[j] + col[i];
-            any |= (sa[t] ^ Q) | (sb[t] ^ Q);
+            const uint32_t w = sa[t];
+            any |= (w ^ Q) | (sb[t] ^ Q);
+            if (j >= SKIP_MARGIN_Y && j < SKIP_MARGIN_Y + CHUNK_H && i >= SKIP_MARGIN_X &&
+                i < SKIP_MARGIN_X + CHUNK_W)
+                act += (w & S_ACTIVE) ? 1u : 0u;
         }
-    __hip_atomic_store(live + c, (uint8_t)(tag << 1 | (any ? 1u : 0u)), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(live + c, (uint8_t)(tag << 5 | act << 1 | (any ? 1u : 0u)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Batched step: every tile of every ensemble member in one launch.  tiles[t]
@@ -1319,15 +1325,20 @@ bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
 {
     const int tiles_x = (nx + TX - 1) / TX, tiles_y = (ny + TY - 1) / TY;
     const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
-    for (int chunk = 0; chunk < ncx * ncy; ++chunk) {
-        const int cy = chunk / ncx;
-        const int cx = (chunk - cy * ncx + CHUNK_SKEW * cy) % ncx;
-        const int c = cy * ncx + cx;
-        const uint8_t f = flags[c];
-        if ((uint32_t)(f >> 1) != tag)
+    // live chunks by their number of active tiles, most first, so that the
+    // launch ends on cheap tiles
+    for (int chunk = 0; chunk < ncx * ncy; ++chunk)
+        if ((uint32_t)(flags[chunk] >> 5) != tag)
             return false;
-        if (f & 1)
-            out.push_back((uint16_t)c);
+    for (int na = CHUNK_W * CHUNK_H; na >= 0; --na) {
+        const uint32_t want = (uint32_t)na << 1 | 1u;
+        for (int chunk = 0; chunk < ncx * ncy; ++chunk) {
+            const int cy = chunk / ncx;
+            const int cx = (chunk - cy * ncx + CHUNK_SKEW * cy) % ncx;
+            const int c = cy * ncx + cx;
+            if ((flags[c] & 31u) == want)
+                out.push_back((uint16_t)c);
+        }
     }
     return true;
 }
