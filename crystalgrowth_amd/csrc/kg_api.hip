@@ -109,21 +109,22 @@ int ensure_status(kg_solver_s &h)
 // SKIP_PERIOD the host (1) takes the liveness answer it asked for SKIP_AGE
 // periods ago, if any, as the list of chunks the next SKIP_PERIOD launches
 // cover (no answer, an answer not all written yet, or more live chunks than
-// a LiveList holds: all chunks), and (2) asks again: k_chunk_live reads the
-// status as the stream stands and writes its flags, tagged with the
-// request's number, to pinned host memory, behind an event.  A chunk is
-// skipped when every tile within SKIP_MARGIN_X tile columns and
-// SKIP_MARGIN_Y tile rows of it (64 and 30 cells) is exactly phi = +0, T =
-// +0 on both buffers with status words saying so; nonzero values cannot
-// reach the chunk in the (SKIP_AGE + 1) * SKIP_PERIOD steps an answer is
-// used for (kg_step.hip checks the margin), and a step of such a tile would
-// write the same zeros and status the buffers hold already, so skipping
-// changes no bit.  The event skips the system-scope fence (a write-back of
-// the L2 that would cost microseconds): the flags do not need it, being
-// written with system-scope stores, and a flag not yet visible shows by its
-// tag.  Waiting on an answer bounds how far the host runs ahead of the GPU
-// to about SKIP_AGE periods.
-int skip_cycle(kg_solver_s &h)
+// a LiveList holds: all chunks), and (2) asks again: the step launch itself
+// carries the request (leading blocks read the input status words and write
+// the flags, tagged with the request's number, to pinned host memory), and
+// an event follows the launch.  A chunk is skipped when every tile within
+// SKIP_MARGIN_X tile columns and SKIP_MARGIN_Y tile rows of it (64 and 30
+// cells) has the status "phi = +0, T = +0" in the state the request read;
+// nonzero values cannot reach the chunk in the (SKIP_AGE + 1) * SKIP_PERIOD
+// steps an answer is used for, and both of its buffers hold those zeros
+// again before the answer is used (kg_step.hip checks the margin and states
+// the argument), so skipping changes no bit.  The event skips the
+// system-scope fence (a write-back of the L2 that would cost microseconds):
+// the flags do not need it, being written with system-scope stores, and a
+// flag not yet visible shows by its tag.  Waiting on an answer bounds how far
+// the host runs ahead of the GPU to about SKIP_AGE + 1 periods.
+// Sets slot to the buffer the request carried by this step writes.
+int skip_cycle(kg_solver_s &h, int &slot)
 {
     const int p = (int)((h.steps / kg::SKIP_PERIOD) % kg::SKIP_AGE);
     h.live.n = -1;
@@ -139,13 +140,8 @@ int skip_cycle(kg_solver_s &h)
         h.live_pending[p] = false;
     }
     h.next_tag = h.next_tag % kg::LIVE_TAGS + 1;
-    KG_HIP(kg::launch_chunk_live(h.stat + (size_t)h.cur * h.stat_tiles,
-                                 h.stat + (size_t)(h.cur ^ 1) * h.stat_tiles, h.tiles_x,
-                                 h.tiles_y, h.params.boundary, h.next_tag, h.live_host[p],
-                                 h.stream));
-    KG_HIP(hipEventRecord(h.live_ev[p], h.stream));
     h.live_tag[p] = h.next_tag;
-    h.live_pending[p] = true;
+    slot = p;
     return KG_OK;
 }
 
@@ -170,8 +166,9 @@ int step_impl(kg_solver_s &h, int nsteps)
         return rc;
     const bool skipping = h.kernel == KG_KERNEL_FUSED && h.stat_ok && h.live_host[0] != nullptr;
     for (int s = 0; s < nsteps; ++s) {
+        int slot = -1;  // >= 0: this launch carries a liveness request
         if (skipping && h.steps % kg::SKIP_PERIOD == 0)
-            if (int rc = skip_cycle(h))
+            if (int rc = skip_cycle(h, slot))
                 return rc;
         kg::StepArgs<R> a = make_args<R>(h);
         a.step = (uint32_t)h.steps;
@@ -183,12 +180,19 @@ int step_impl(kg_solver_s &h, int nsteps)
                                base + 5 * n};
             e = kg::launch_step_two_pass<R>(a, t, h.stream);
         } else {
-            e = kg::launch_step_fused<R>(a, h.stream, skipping ? &h.live : nullptr);
+            const kg::LiveReq req{slot >= 0 ? h.live_host[slot] : nullptr,
+                                  slot >= 0 ? h.live_tag[slot] : 0u, 0};
+            e = kg::launch_step_fused<R>(a, h.stream, skipping ? &h.live : nullptr,
+                                         slot >= 0 ? &req : nullptr);
             if (skipping && h.live.n >= 0 && h.live.n < h.nchunks)
                 ++h.listed_launches;
         }
         if (e != hipSuccess)
             return hip_fail(e, "step launch");
+        if (slot >= 0) {
+            KG_HIP(hipEventRecord(h.live_ev[slot], h.stream));
+            h.live_pending[slot] = true;
+        }
         h.cur ^= 1;
         h.steps += 1;
     }

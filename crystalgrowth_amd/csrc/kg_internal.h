@@ -62,7 +62,7 @@ struct ReduceOut {
 // cx) in the order the launch deals them to the XCDs.  Every chunk left out
 // holds phi = +0 and T = +0 on both buffers and keeps them over the launch,
 // and its status words already say so (the host derives the list from
-// launch_chunk_live; kg_api.hip states the conditions).  Passed by value in
+// a liveness request; kg_api.hip states the conditions).  Passed by value in
 // the kernel arguments: the launch has only the listed chunks' blocks.
 constexpr int LIVE_MAX = 1020;
 struct LiveList {
@@ -70,11 +70,28 @@ struct LiveList {
     alignas(4) uint16_t c[LIVE_MAX];  // read by the kernel as 32-bit words
 };
 static_assert(LIVE_MAX % 2 == 0, "LiveList entries are read in pairs");
-// live == nullptr or live->n < 0: every chunk
+// A liveness request carried by a step launch: leading blocks of the launch
+// write one flag byte per chunk to `flags` (pinned host memory, one byte per
+// chunk, each written with a system-scope store):
+//   tag << 5 | na << 1 | lv,
+// lv = 1 if some tile within SKIP_MARGIN_X tile columns and SKIP_MARGIN_Y
+// tile rows of the chunk (mapped by the boundary) has an input status word
+// other than exactly "phi = +0, T = +0"; na = tiles of the chunk whose last
+// step took the full path; tag in 1..LIVE_TAGS.  `blocks` is set by
+// launch_step_fused.
+struct LiveReq {
+    uint8_t *flags;
+    uint32_t tag;
+    int blocks;
+};
+constexpr uint32_t LIVE_TAGS = 7;
+// live == nullptr or live->n < 0: every chunk; req != nullptr (and the tile
+// status on): the launch also answers a liveness request
 template <typename R>
-hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live = nullptr);
+hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live = nullptr,
+                             const LiveReq *req = nullptr);
 // Append to out the chunks flagged live in flags[c] (one byte per chunk, as
-// launch_chunk_live writes it), in the order a LiveList must hold them: by
+// a LiveReq writes it), in the order a LiveList must hold them: by
 // their count of active tiles, most first (a launch then ends on cheap
 // tiles), the full launch's chunk order within one count.  False, with out
 // incomplete, if some byte does not carry `tag` (not yet written by the
@@ -84,8 +101,8 @@ bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
 // Chunk grid of the fused step for an nx x ny grid (chunks_x * chunks_y chunks).
 int fused_chunks_x(int nx);
 int fused_chunks_y(int ny);
-// Tile rows/columns of margin around a chunk that launch_chunk_live checks,
-// and the period (steps) of the host's liveness requests.
+// Tile rows/columns of margin around a chunk that a liveness request checks,
+// and the period (steps) of the host's requests.
 constexpr int SKIP_MARGIN_X = 1, SKIP_MARGIN_Y = 1;  // tile columns, tile rows
 constexpr int SKIP_PERIOD = 7;
 // periods between a liveness request and the first launch that uses it
@@ -95,16 +112,6 @@ constexpr int SKIP_AGE = 1;
 // two periods of the GPU, so handles stepped in turn would stop overlapping
 // on the GPU (256 chunks of 2 x 4 tiles: about 2000 x 2000 cells)
 constexpr int SKIP_MIN_CHUNKS = 256;
-// live[c] = tag << 5 | na << 1 | lv: lv = 1 if some tile within
-// SKIP_MARGIN_X tile columns and SKIP_MARGIN_Y tile rows of chunk c (mapped
-// by the boundary) has a status word other than exactly "phi = +0, T = +0"
-// in either set (stat_a, stat_b); na = tiles of chunk c whose stat_a word
-// says the last step took the full path; tag in 1..LIVE_TAGS.  live is host
-// memory; each byte is written with a system-scope store.
-constexpr uint32_t LIVE_TAGS = 7;
-hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,
-                             int tiles_y, int boundary, uint32_t tag, uint8_t *live,
-                             hipStream_t s);
 template <typename R>
 hipError_t launch_step_two_pass(const StepArgs<R> &a, const PassTemps<R> &t, hipStream_t s);
 

@@ -239,11 +239,11 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
 // (below) the smaller chunk skips more: 40.7 us at 2 x 8, 40.1 us at 2 x 4.
 constexpr int CHUNK_W = 2, CHUNK_H = 4, CHUNK_SKEW = 3;
 
-// Tile (tx, ty) of this block and its chunk's row-major index c; false for
-// blocks past the grid.
-__device__ __forceinline__ bool chunk_tile(int tiles_x, int tiles_y, int &tx, int &ty, int &c)
+// Tile (tx, ty) of tile block b and its chunk's row-major index c; false
+// for blocks past the grid.
+__device__ __forceinline__ bool chunk_tile(int b, int tiles_x, int tiles_y, int &tx, int &ty,
+                                           int &c)
 {
-    const int b = blockIdx.x;
     const int i = b / NXCD;
     const int chunk = (i / (CHUNK_W * CHUNK_H)) * NXCD + b % NXCD;
     const int k = i % (CHUNK_W * CHUNK_H);
@@ -885,13 +885,12 @@ template <typename R> struct Occupancy;
 template <> struct Occupancy<float> { static constexpr int waves = KG_WAVES_PER_EU; };
 template <> struct Occupancy<double> { static constexpr int waves = 1; };
 
-// Tile of this block when the launch covers only the chunks of `live`: the
-// j-th listed chunk goes to XCD j % 8 and its 16 tiles to consecutive
+// Tile of tile block b when the launch covers only the chunks of `live`:
+// the j-th listed chunk goes to XCD j % 8 and its 8 tiles to consecutive
 // blocks of that XCD, as chunk_tile deals all chunks.
-__device__ __forceinline__ bool listed_tile(const LiveList &live, int tiles_x, int tiles_y, int &tx,
-                                            int &ty)
+__device__ __forceinline__ bool listed_tile(const LiveList &live, int b, int tiles_x, int tiles_y,
+                                            int &tx, int &ty)
 {
-    const int b = blockIdx.x;
     const int i = b / NXCD;
     const int j = (i / (CHUNK_W * CHUNK_H)) * NXCD + b % NXCD;
     if (j >= live.n)
@@ -906,13 +905,69 @@ __device__ __forceinline__ bool listed_tile(const LiveList &live, int tiles_x, i
     return tx < tiles_x && ty < tiles_y;
 }
 
+// Liveness request folded into a step launch (every SKIP_PERIOD steps): the
+// launch's first q.blocks blocks (a multiple of 8, so that the tile blocks
+// keep their XCDs) take NT chunks each, one thread per chunk.  A thread
+// reads the input status words of its chunk's tiles and of a margin of
+// SKIP_MARGIN_X tile columns and SKIP_MARGIN_Y tile rows (mapped by the
+// boundary), all loads in flight together, and stores the flag byte
+// tag << 5 | na << 1 | lv (lv: some word is not exactly "phi = +0, T = +0";
+// na: tiles of the chunk whose last step took the full path) with a
+// system-scope store, straight into the host buffer; the host checks the
+// tag, so no fence has to write back the L2.  The launch writes only the
+// other status set, so the words read are those of the state it starts from.
+constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_X, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_Y;
+static_assert(CHUNK_W * CHUNK_H < 16, "active-tile count fits 4 bits");
+__device__ __forceinline__ int live_wrap(int v, int n, int boundary)
+{
+    if (boundary == 1)
+        return v < 0 ? 0 : (v >= n ? n - 1 : v);
+    return ((v % n) + n) % n;
+}
+template <typename R>
+__device__ __forceinline__ void live_request(const StepArgs<R> &a, const LiveReq &q, int tiles_x,
+                                             int tiles_y)
+{
+    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
+    const int c = blockIdx.x * NT + threadIdx.x;
+    if (c >= ncx * ncy)
+        return;
+    const int cx = c % ncx, cy = c / ncx;
+    constexpr uint32_t Q = S_PHI0 | S_T0;
+    const uint32_t *__restrict__ sa = a.stat_in + a.tile_base;
+    int col[LIVE_W], row[LIVE_H];
+#pragma unroll
+    for (int i = 0; i < LIVE_W; ++i)
+        col[i] = live_wrap(cx * CHUNK_W - SKIP_MARGIN_X + i, tiles_x, a.boundary);
+#pragma unroll
+    for (int j = 0; j < LIVE_H; ++j)
+        row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y + j, tiles_y, a.boundary) * tiles_x;
+    uint32_t any = 0, act = 0;
+#pragma unroll
+    for (int j = 0; j < LIVE_H; ++j)
+#pragma unroll
+        for (int i = 0; i < LIVE_W; ++i) {
+            const uint32_t w = sa[row[j] + col[i]];
+            any |= w ^ Q;
+            if (j >= SKIP_MARGIN_Y && j < SKIP_MARGIN_Y + CHUNK_H && i >= SKIP_MARGIN_X &&
+                i < SKIP_MARGIN_X + CHUNK_W)
+                act += (w & S_ACTIVE) ? 1u : 0u;
+        }
+    __hip_atomic_store(q.flags + c, (uint8_t)(q.tag << 5 | act << 1 | (any ? 1u : 0u)),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename R, int BC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a, LiveReq q)
 {
     __shared__ __align__(16) TileSmem<R> sm;
     const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+    if ((int)blockIdx.x < q.blocks) {
+        live_request(a, q, tiles_x, tiles_y);
+        return;
+    }
     int tx, ty, c;
-    if (!chunk_tile(tiles_x, tiles_y, tx, ty, c))
+    if (!chunk_tile((int)blockIdx.x - q.blocks, tiles_x, tiles_y, tx, ty, c))
         return;
     step_tile<R, BC>(a, tx * TX, ty * TY, sm);
 }
@@ -920,12 +975,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 // The same step over the chunks of `live` only (a separate kernel, so that
 // launches of every chunk keep their small kernel arguments).
 template <typename R, int BC>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused_listed(StepArgs<R> a, LiveList live)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused_listed(StepArgs<R> a, LiveList live, LiveReq q)
 {
     __shared__ __align__(16) TileSmem<R> sm;
     const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+    if ((int)blockIdx.x < q.blocks) {
+        live_request(a, q, tiles_x, tiles_y);
+        return;
+    }
     int tx, ty;
-    if (!listed_tile(live, tiles_x, tiles_y, tx, ty))
+    if (!listed_tile(live, (int)blockIdx.x - q.blocks, tiles_x, tiles_y, tx, ty))
         return;
     step_tile<R, BC>(a, tx * TX, ty * TY, sm);
 }
@@ -933,64 +992,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 // A chunk skipped by a launch must stay all +0 for the launch.  The host uses
 // a liveness answer for the SKIP_PERIOD steps that start SKIP_AGE periods
 // after it asked, so up to (SKIP_AGE + 1) SKIP_PERIOD steps separate the
-// status it saw from the last step that uses it.  A step moves nonzero values at most 2 cells
-// (pass 2 reads pass-1 values of neighbours, which read phi one cell
-// further), and a tile reads 2 cells beyond itself; the margin must cover
-// both.
-static_assert(CHUNK_W * CHUNK_H < 16, "active-tile count fits 4 bits");
+// state whose status the request read from the last step that uses it.  A
+// step moves nonzero values at most 2 cells (pass 2 reads pass-1 values of
+// neighbours, which read phi one cell further), and a tile reads 2 cells
+// beyond itself; the margin must cover both.  The request reads the input
+// status set only: by the first launch that uses the answer, SKIP_AGE >= 1
+// periods later, both buffers of the chunk have been rewritten with states
+// the margin keeps all +0 there (or left alone by launches that skipped the
+// chunk under an earlier answer, which held the same zeros), so a skipped
+// launch's output buffer already holds what the step would write.
+static_assert(SKIP_AGE >= 1, "the request reads only the input status set");
 static_assert(2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_Y * TY &&
                   2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_X * TX,
               "chunk skip margin too small for the request period");
-
-// Liveness of the chunks (see launch_chunk_live): one thread per chunk, its
-// tile rows and columns (margin included) mapped onto the grid first, then
-// every status load in flight together.  Flag byte = tag << 5 | active
-// tiles << 1 | live, stored with a system-scope store so that it goes
-// straight to the host buffer (the host checks the tag, so it needs no fence
-// that writes back the L2).
-constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_X, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_Y;
-constexpr int LIVE_THREADS = 64;
-__device__ __forceinline__ int live_wrap(int v, int n, int boundary)
-{
-    if (boundary == 1)
-        return v < 0 ? 0 : (v >= n ? n - 1 : v);
-    return ((v % n) + n) % n;
-}
-__global__ __launch_bounds__(LIVE_THREADS) void k_chunk_live(const uint32_t *__restrict__ sa,
-                                                             const uint32_t *__restrict__ sb,
-                                                             int tiles_x, int tiles_y,
-                                                             int boundary, int nchunks,
-                                                             uint32_t tag,
-                                                             uint8_t *__restrict__ live)
-{
-    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
-    const int c = blockIdx.x * LIVE_THREADS + threadIdx.x;
-    if (c >= nchunks)
-        return;
-    const int cx = c % ncx, cy = c / ncx;
-    constexpr uint32_t Q = S_PHI0 | S_T0;
-    int col[LIVE_W], row[LIVE_H];
-#pragma unroll
-    for (int i = 0; i < LIVE_W; ++i)
-        col[i] = live_wrap(cx * CHUNK_W - SKIP_MARGIN_X + i, tiles_x, boundary);
-#pragma unroll
-    for (int j = 0; j < LIVE_H; ++j)
-        row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y + j, tiles_y, boundary) * tiles_x;
-    uint32_t any = 0, act = 0;  // act: tiles of the chunk that took the full path
-#pragma unroll
-    for (int j = 0; j < LIVE_H; ++j)
-#pragma unroll
-        for (int i = 0; i < LIVE_W; ++i) {
-            const int t = row[j] + col[i];
-            const uint32_t w = sa[t];
-            any |= (w ^ Q) | (sb[t] ^ Q);
-            if (j >= SKIP_MARGIN_Y && j < SKIP_MARGIN_Y + CHUNK_H && i >= SKIP_MARGIN_X &&
-                i < SKIP_MARGIN_X + CHUNK_W)
-                act += (w & S_ACTIVE) ? 1u : 0u;
-        }
-    __hip_atomic_store(live + c, (uint8_t)(tag << 5 | act << 1 | (any ? 1u : 0u)),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 
 // Batched step: every tile of every ensemble member in one launch.  tiles[t]
 // = (member, x0, y0); members[m] holds the member's fields (buffer 0 as
@@ -1294,27 +1308,36 @@ unsigned grid_1d(size_t n, unsigned threads)
 int fused_tiles(int nx, int ny) { return ((nx + TX - 1) / TX) * ((ny + TY - 1) / TY); }
 
 template <typename R>
-hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live)
+hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList *live,
+                             const LiveReq *req)
 {
     StepArgs<R> b = a;
     b.ntiles = fused_tiles(a.nx, a.ny);
     const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
     const int nchunks = ((tiles_x + CHUNK_W - 1) / CHUNK_W) * ((tiles_y + CHUNK_H - 1) / CHUNK_H);
+    LiveReq q{nullptr, 0u, 0};
+    if (req && a.stat_in) {
+        q = *req;
+        q.blocks = (nchunks + NT - 1) / NT;
+        q.blocks = (q.blocks + NXCD - 1) / NXCD * NXCD;  // tile blocks keep their XCDs
+    }
     if (live && live->n >= 0 && live->n < nchunks) {
-        const int nblocks = (live->n + NXCD - 1) / NXCD * NXCD * CHUNK_W * CHUNK_H;
+        const int nblocks = (live->n + NXCD - 1) / NXCD * NXCD * CHUNK_W * CHUNK_H + q.blocks;
         if (nblocks == 0)
-            return hipSuccess;  // no chunk to step
+            return hipSuccess;  // no chunk to step, nothing asked
         if (a.boundary == 1)
-            hipLaunchKernelGGL((k_step_fused_listed<R, 1>), dim3(nblocks), dim3(NT), 0, s, b, *live);
+            hipLaunchKernelGGL((k_step_fused_listed<R, 1>), dim3(nblocks), dim3(NT), 0, s, b, *live,
+                               q);
         else
-            hipLaunchKernelGGL((k_step_fused_listed<R, 0>), dim3(nblocks), dim3(NT), 0, s, b, *live);
+            hipLaunchKernelGGL((k_step_fused_listed<R, 0>), dim3(nblocks), dim3(NT), 0, s, b, *live,
+                               q);
         return hipGetLastError();
     }
-    const int nblocks = chunk_blocks(tiles_x, tiles_y);
+    const int nblocks = chunk_blocks(tiles_x, tiles_y) + q.blocks;
     if (a.boundary == 1)
-        hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(nblocks), dim3(NT), 0, s, b);
+        hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(nblocks), dim3(NT), 0, s, b, q);
     else
-        hipLaunchKernelGGL((k_step_fused<R, 0>), dim3(nblocks), dim3(NT), 0, s, b);
+        hipLaunchKernelGGL((k_step_fused<R, 0>), dim3(nblocks), dim3(NT), 0, s, b, q);
     return hipGetLastError();
 }
 
@@ -1345,17 +1368,6 @@ bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
 
 int fused_chunks_x(int nx) { return ((nx + TX - 1) / TX + CHUNK_W - 1) / CHUNK_W; }
 int fused_chunks_y(int ny) { return ((ny + TY - 1) / TY + CHUNK_H - 1) / CHUNK_H; }
-
-hipError_t launch_chunk_live(const uint32_t *stat_a, const uint32_t *stat_b, int tiles_x,
-                             int tiles_y, int boundary, uint32_t tag, uint8_t *live,
-                             hipStream_t s)
-{
-    const int n = ((tiles_x + CHUNK_W - 1) / CHUNK_W) * ((tiles_y + CHUNK_H - 1) / CHUNK_H);
-    hipLaunchKernelGGL(k_chunk_live, dim3((n + LIVE_THREADS - 1) / LIVE_THREADS),
-                       dim3(LIVE_THREADS), 0, s, stat_a, stat_b, tiles_x, tiles_y, boundary, n,
-                       tag, live);
-    return hipGetLastError();
-}
 
 int fused_tile_x() { return TX; }
 int fused_tile_y() { return TY; }
@@ -1512,7 +1524,8 @@ hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double 
 }
 
 #define KG_INSTANTIATE(R)                                                                        \
-    template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t, const LiveList *); \
+    template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t, const LiveList *, \
+                                             const LiveReq *);                                \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
                                              const uint32_t *, uint32_t, hipStream_t);           \
     template hipError_t launch_step_batch_direct<R>(const StepArgs<R> *, const int4 *, int,      \
