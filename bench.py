@@ -13,8 +13,8 @@ the configuration's full horizon), with the fields resident in HBM.
 Prints one JSON line.  `value` is cell-updates per second of the timed
 region; `roofline` prices the step's average duration on the GPU (HIP events
 on the solver's own stream around the timed steps: each step is one launch
-of the fused kernel, over every 2 x 4-tile chunk or over the live ones, plus
-the chunk liveness kernel every 7 steps) at 16 algorithmic bytes per
+of the fused kernel, over every 2 x 4-tile chunk or over the live ones; every
+7th launch also answers the host's liveness request) at 16 algorithmic bytes per
 cell-update against the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the reference's
 arithmetic restated in C, one thread) on a bounded sample of the same
 workload (`cpu_all_cores`: the same oracle on OpenMP threads, labelled
@@ -371,7 +371,7 @@ def main():
                      "traffic": traffic_from_profiles(n),
                      "kernel": ("k_pass1+k_pass2" if args.kernel == "two_pass" else
                                 "k_step_fused<float> (all chunks) or k_step_fused_listed<float> "
-                                "(live chunks), + k_chunk_live every 7 steps"),
+                                "(live chunks, heaviest first; every 7th launch also answers the liveness request)"),
                      "avg_launch_us": round(ms_per_step * 1e3, 3),
                      "alg_bytes_per_cell_update": ALG_BYTES_FP32},
         "final_state": {"solid_fraction": st["solid_fraction"], "sum_phi": st["sum_phi"],
