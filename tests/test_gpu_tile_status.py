@@ -172,3 +172,17 @@ def test_chunk_skipping_after_new_fields(kb):
             s.step(3)
             s.step(30)
         assert_bitwise([(74, a.fields(), b.fields())])
+
+
+def test_headline_trajectory_matches_dense_bitwise(kb):
+    """The 4096 x 4096 workload of the bench line (zero-flux, noise 0.01,
+    nine r = 5 seeds) over its 20200 steps: every skip (tile status, chunk
+    lists ordered heaviest first, liveness requests carried by the step
+    launches) leaves the final fields bitwise those of the dense kernel."""
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                          boundary=1, seed=0)
+    stats = []
+    pairs = run_pair(kb, 4096, 4096, "f32", p, lambda s: s.reset_seeds(9, 5.0, 8, 0),
+                     [200, 20200], stats)
+    assert_bitwise(pairs)
+    assert stats[0]["listed_launches"] >= 20000 and stats[0]["stale_answers"] == 0, stats
