@@ -916,7 +916,15 @@ __device__ __forceinline__ bool listed_tile(const LiveList &live, int b, int til
 // system-scope store, straight into the host buffer; the host checks the
 // tag, so no fence has to write back the L2.  The launch writes only the
 // other status set, so the words read are those of the state it starts from.
-constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_X, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_Y;
+// Cells a margin must span (see the static_assert below the kernels).  A
+// periodic grid's last tile column or row can be narrower: where a margin
+// tile is that partial tile, the window takes the tile beyond it as well
+// (a full tile spans enough); elsewhere that extra slot repeats the margin
+// tile, adding no condition.
+constexpr int LIVE_NEED = 2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2;
+static_assert(SKIP_MARGIN_X == 1 && SKIP_MARGIN_Y == 1 && LIVE_NEED <= TX && LIVE_NEED <= TY,
+              "one full margin tile spans a margin; the window logic assumes one");
+constexpr int LIVE_W = CHUNK_W + 2 * SKIP_MARGIN_X + 2, LIVE_H = CHUNK_H + 2 * SKIP_MARGIN_Y + 2;
 static_assert(CHUNK_W * CHUNK_H < 16, "active-tile count fits 4 bits");
 __device__ __forceinline__ int live_wrap(int v, int n, int boundary)
 {
@@ -938,19 +946,30 @@ __device__ __forceinline__ void live_request(const StepArgs<R> &a, const LiveReq
     int col[LIVE_W], row[LIVE_H];
 #pragma unroll
     for (int i = 0; i < LIVE_W; ++i)
-        col[i] = live_wrap(cx * CHUNK_W - SKIP_MARGIN_X + i, tiles_x, a.boundary);
+        col[i] = live_wrap(cx * CHUNK_W - SKIP_MARGIN_X - 1 + i, tiles_x, a.boundary);
 #pragma unroll
     for (int j = 0; j < LIVE_H; ++j)
-        row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y + j, tiles_y, a.boundary) * tiles_x;
-    uint32_t any = 0, act = 0;
+        row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y - 1 + j, tiles_y, a.boundary);
+    const int wx = a.nx - (tiles_x - 1) * TX, wy = a.ny - (tiles_y - 1) * TY;
+    const bool narrow_x = a.boundary == 0 && wx < LIVE_NEED;
+    const bool narrow_y = a.boundary == 0 && wy < LIVE_NEED;
+    if (!(narrow_x && col[1] == tiles_x - 1))
+        col[0] = col[1];
+    if (!(narrow_x && col[LIVE_W - 2] == tiles_x - 1))
+        col[LIVE_W - 1] = col[LIVE_W - 2];
+    if (!(narrow_y && row[1] == tiles_y - 1))
+        row[0] = row[1];
+    if (!(narrow_y && row[LIVE_H - 2] == tiles_y - 1))
+        row[LIVE_H - 1] = row[LIVE_H - 2];
+    uint32_t any = 0, act = 0;  // act: tiles of the chunk that took the full path
 #pragma unroll
     for (int j = 0; j < LIVE_H; ++j)
 #pragma unroll
         for (int i = 0; i < LIVE_W; ++i) {
-            const uint32_t w = sa[row[j] + col[i]];
+            const uint32_t w = sa[row[j] * tiles_x + col[i]];
             any |= w ^ Q;
-            if (j >= SKIP_MARGIN_Y && j < SKIP_MARGIN_Y + CHUNK_H && i >= SKIP_MARGIN_X &&
-                i < SKIP_MARGIN_X + CHUNK_W)
+            if (j > SKIP_MARGIN_Y && j <= SKIP_MARGIN_Y + CHUNK_H && i > SKIP_MARGIN_X &&
+                i <= SKIP_MARGIN_X + CHUNK_W)
                 act += (w & S_ACTIVE) ? 1u : 0u;
         }
     __hip_atomic_store(q.flags + c, (uint8_t)(q.tag << 5 | act << 1 | (any ? 1u : 0u)),

@@ -186,3 +186,19 @@ def test_headline_trajectory_matches_dense_bitwise(kb):
                      [200, 20200], stats)
     assert_bitwise(pairs)
     assert stats[0]["listed_launches"] >= 20000 and stats[0]["stale_answers"] == 0, stats
+
+
+@pytest.mark.parametrize("boundary", [0, 1])
+def test_chunk_skipping_on_ragged_chunk_grid(kb, boundary):
+    """2050 x 1930 cells: 33 x 65 tiles, the last tile column 2 cells wide and
+    the last tile row 10 rows high, so the last chunk column and row are
+    partial (17 x 17 chunks of 2 x 4 tiles).  Nuclei next to the far corner
+    grow into the partial chunks; periodic fields wrap across it."""
+    nx, ny = 2050, 1930
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                          boundary=boundary, seed=8)
+    stats = []
+    pairs = run_pair(kb, nx, ny, "f32", p, lambda s: s.reset_disk(nx - 20, ny - 15, 6.0),
+                     [1, 8, 29, 150, 400], stats)
+    assert_bitwise(pairs)
+    assert stats[0]["listed_launches"] >= 350 and stats[0]["stale_answers"] == 0, stats
