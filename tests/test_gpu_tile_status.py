@@ -189,12 +189,14 @@ def test_headline_trajectory_matches_dense_bitwise(kb):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
-def test_chunk_skipping_on_ragged_chunk_grid(kb, boundary):
+@pytest.mark.parametrize("nx,ny", [(2050, 1930), (2051, 1931)])
+def test_chunk_skipping_on_ragged_chunk_grid(kb, boundary, nx, ny):
     """2050 x 1930 cells: 33 x 65 tiles, the last tile column 2 cells wide and
     the last tile row 10 rows high, so the last chunk column and row are
-    partial (17 x 17 chunks of 2 x 4 tiles).  Nuclei next to the far corner
-    grow into the partial chunks; periodic fields wrap across it."""
-    nx, ny = 2050, 1930
+    partial (17 x 17 chunks of 2 x 4 tiles).  A nucleus next to the far
+    corner grows into the partial chunks; periodic fields wrap across it.
+    2051 x 1931 (3 and 11 cells) also takes the cell-by-cell staging of
+    widths that are not a multiple of 4."""
     p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
                           boundary=boundary, seed=8)
     stats = []
