@@ -604,6 +604,11 @@ int kg_set_step_count(kg_handle h, uint64_t steps)
     if (int rc = check_handle(h))
         return rc;
     h->steps = steps;
+    // the liveness cycle runs on the step count: a list or answer in hand
+    // could now be used beyond the steps its margin covers
+    h->live.n = -1;
+    for (bool &q : h->live_pending)
+        q = false;
     return KG_OK;
 }
 

@@ -204,3 +204,23 @@ def test_chunk_skipping_on_ragged_chunk_grid(kb, boundary, nx, ny):
                      [1, 8, 29, 150, 400], stats)
     assert_bitwise(pairs)
     assert stats[0]["listed_launches"] >= 350 and stats[0]["stale_answers"] == 0, stats
+
+
+def test_chunk_skipping_across_step_count_changes(kb):
+    """The step count (the noise counter) is set between calls, off the
+    liveness period, while an answer is in flight and a list is in use; the
+    status solver must still match the dense one bitwise."""
+    nx, ny = 2048, 1920
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                          boundary=0, seed=9)
+    with kb.Solver(nx, ny, "f32", p) as a, kb.Solver(nx, ny, "f32", p) as b:
+        b.set_kernel("fused_dense")
+        for s in (a, b):
+            s.reset_disk(1000, 900, 8.0)
+        for count, n in ((None, 60), (1003, 9), (5, 12), (77777, 40)):
+            for s in (a, b):
+                if count is not None:
+                    s.step_count = count
+                s.step(n)
+        assert_bitwise([(121, a.fields(), b.fields())])
+        assert a.skip_stats()["listed_launches"] > 0
