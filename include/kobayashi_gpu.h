@@ -134,12 +134,14 @@ int kg_step_timed(kg_handle h, int nsteps, double *ms);
 int kg_synchronize(kg_handle h);
 /* Steps taken since the last reset (the noise counter of the next step). */
 int kg_get_step_count(kg_handle h, uint64_t *steps);
+/* Sets that counter (the noise counter); the next steps cover every chunk
+ * until a fresh answer on which regions stay all +0 arrives. */
+int kg_set_step_count(kg_handle h, uint64_t steps);
 /* Diagnostic, no counterpart in the reference: step launches since kg_create
  * that covered only the chunks listed as live (the rest all +0), and liveness
  * answers dropped because a flag was not yet visible to the host.  Chunk
  * lists are used on grids of at least 256 chunks of 128 x 120 cells. */
 int kg_get_skip_stats(kg_handle h, uint64_t *listed_launches, uint64_t *stale_answers);
-int kg_set_step_count(kg_handle h, uint64_t steps);
 int kg_set_kernel(kg_handle h, int kernel);
 
 /* ---- fields --------------------------------------------------------------------- */
