@@ -189,8 +189,9 @@ def test_headline_trajectory_matches_dense_bitwise(kb):
 
 
 @pytest.mark.parametrize("boundary", [0, 1])
-@pytest.mark.parametrize("nx,ny", [(2050, 1930), (2051, 1931)])
-def test_chunk_skipping_on_ragged_chunk_grid(kb, boundary, nx, ny):
+@pytest.mark.parametrize("nx,ny,prec", [(2050, 1930, "f32"), (2051, 1931, "f32"),
+                                        (2050, 1930, "f64")])
+def test_chunk_skipping_on_ragged_chunk_grid(kb, boundary, nx, ny, prec):
     """2050 x 1930 cells: 33 x 65 tiles, the last tile column 2 cells wide and
     the last tile row 10 rows high, so the last chunk column and row are
     partial (17 x 17 chunks of 2 x 4 tiles).  A nucleus next to the far
@@ -200,7 +201,7 @@ def test_chunk_skipping_on_ragged_chunk_grid(kb, boundary, nx, ny):
     p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
                           boundary=boundary, seed=8)
     stats = []
-    pairs = run_pair(kb, nx, ny, "f32", p, lambda s: s.reset_disk(nx - 20, ny - 15, 6.0),
+    pairs = run_pair(kb, nx, ny, prec, p, lambda s: s.reset_disk(nx - 20, ny - 15, 6.0),
                      [1, 8, 29, 150, 400], stats)
     assert_bitwise(pairs)
     assert stats[0]["listed_launches"] >= 350 and stats[0]["stale_answers"] == 0, stats
