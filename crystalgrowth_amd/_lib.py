@@ -144,6 +144,7 @@ SIGNATURES = {
     "kg_batch_step_timed": [H, I, ctypes.POINTER(ctypes.c_double)],
     "kg_batch_synchronize": [H],
     "kg_batch_get_step_count": [H, ctypes.POINTER(U64)],
+    "kg_batch_get_skip_stats": [H, ctypes.POINTER(U64), ctypes.POINTER(U64)],
 }
 
 

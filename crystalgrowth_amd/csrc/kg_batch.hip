@@ -56,10 +56,17 @@ void free_batch(kg_batch_s *b)
     kg::free_adjoint(b->adj);
     b->adj = nullptr;
     void *bufs[] = {b->phi[0], b->phi[1], b->T[0], b->T[1], b->theta, b->members,
-                    b->tiles, b->step_base, b->partials, b->red, b->disks, b->stat};
+                    b->tiles, b->step_base, b->partials, b->red, b->disks, b->stat,
+                    b->chunks_dev, b->tiles_listed};
     for (void *p : bufs)
         if (p)
             (void)hipFree(p);
+    if (b->live_flags)
+        (void)hipHostFree(b->live_flags);
+    if (b->tiles_stage)
+        (void)hipHostFree(b->tiles_stage);
+    if (b->live_ev)
+        (void)hipEventDestroy(b->live_ev);
     if (b->ev0)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
@@ -76,37 +83,61 @@ void free_batch(kg_batch_s *b)
 // walks its chunks one after the other, and its sequence is padded to the
 // longest one with empty entries (member -1).  So every XCD steps tiles of
 // many members at any time, and tiles that share halos meet in one L2.
-// *real = number of tiles (entries that are not padding).
-std::vector<int4> batch_tile_table(const int *nx, const int *ny, int count, size_t *real)
+constexpr int BCW = 2, BCH = 4, BSKEW = 3, BNX = 8;
+
+// the chunks of every member in that order: (member, chunk column, chunk row)
+std::vector<int4> batch_chunks(const int *nx, const int *ny, int count)
 {
-    constexpr int NX = 8, CW = 2, CH = 4, SKEW = 3;
     const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
-    std::vector<std::vector<int4>> seq(NX);
-    size_t chunk = 0, n = 0;
+    std::vector<int4> out;
     for (int m = 0; m < count; ++m) {
         const int tx = (nx[m] + TX - 1) / TX, ty = (ny[m] + TY - 1) / TY;
-        const int ncx = (tx + CW - 1) / CW, ncy = (ty + CH - 1) / CH;
+        const int ncx = (tx + BCW - 1) / BCW, ncy = (ty + BCH - 1) / BCH;
         for (int cy = 0; cy < ncy; ++cy)
-            for (int c = 0; c < ncx; ++c, ++chunk) {
-                const int cx = (c + SKEW * cy) % ncx;
-                for (int k = 0; k < CW * CH; ++k) {
-                    const int x = cx * CW + k % CW, y = cy * CH + k / CW;
-                    if (x < tx && y < ty) {
-                        seq[chunk % NX].push_back(make_int4(m, x * TX, y * TY, 0));
-                        ++n;
-                    }
-                }
+            for (int c = 0; c < ncx; ++c)
+                out.push_back(make_int4(m, (c + BSKEW * cy) % ncx, cy, 0));
+    }
+    return out;
+}
+
+// The tile table of chunks[order[0]], chunks[order[1]], ...: the j-th goes
+// to XCD j % 8.  *real = number of tiles (entries that are not padding).
+std::vector<int4> deal_chunks(const std::vector<int4> &chunks, const std::vector<int> &order,
+                              const int *nx, const int *ny, size_t *real)
+{
+    const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
+    std::vector<std::vector<int4>> seq(BNX);
+    size_t n = 0;
+    for (size_t j = 0; j < order.size(); ++j) {
+        const int4 ch = chunks[(size_t)order[j]];
+        const int m = ch.x;
+        const int tx = (nx[m] + TX - 1) / TX, ty = (ny[m] + TY - 1) / TY;
+        for (int k = 0; k < BCW * BCH; ++k) {
+            const int x = ch.y * BCW + k % BCW, y = ch.z * BCH + k / BCW;
+            if (x < tx && y < ty) {
+                seq[j % BNX].push_back(make_int4(m, x * TX, y * TY, 0));
+                ++n;
             }
+        }
     }
     size_t len = 0;
     for (const auto &q : seq)
         len = std::max(len, q.size());
-    std::vector<int4> table(len * NX, make_int4(-1, 0, 0, 0));
-    for (int x = 0; x < NX; ++x)
+    std::vector<int4> table(len * BNX, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < BNX; ++x)
         for (size_t i = 0; i < seq[x].size(); ++i)
-            table[i * NX + x] = seq[x][i];
+            table[i * BNX + x] = seq[x][i];
     *real = n;
     return table;
+}
+
+std::vector<int4> batch_tile_table(const int *nx, const int *ny, int count, size_t *real)
+{
+    const std::vector<int4> ch = batch_chunks(nx, ny, count);
+    std::vector<int> order(ch.size());
+    for (size_t j = 0; j < order.size(); ++j)
+        order[j] = (int)j;
+    return deal_chunks(ch, order, nx, ny, real);
 }
 
 template <typename R>
@@ -143,12 +174,80 @@ std::vector<kg::StepArgs<R>> member_args(const kg_batch_s &b)
 // The tile status no longer describes the buffers: cleared before the next step.
 void invalidate_status(kg_batch_s &b) { b.stat_valid = false; }
 
+// Answers and lists about other fields or steps are void.
+void drop_lists(kg_batch_s &b)
+{
+    b.ntiles_listed = -1;
+    b.live_pending = false;
+}
+
 int ensure_status(kg_batch_s &b)
 {
     if (b.stat_valid)
         return KG_OK;
     KG_HIP(kg::launch_fill_u32(b.stat, 2 * (size_t)b.stat_tiles, 0u, b.stream));
     b.stat_valid = true;
+    drop_lists(b);
+    return KG_OK;
+}
+
+// The chunk lists of the plain solver (kg_api.hip, skip_cycle) over the
+// chunks of every member: every SKIP_PERIOD steps the host takes the answer
+// it asked for one period ago as the listed chunks' tile table (live chunks
+// by active-tile count, most first, dealt to the XCDs as the full table;
+// copied to the device behind the launches that use the previous one) and
+// asks again with k_batch_live on the input status set of the next step.
+// Each member's chunk keeps its margin inside the member (its boundary map),
+// so the plain solver's exactness argument holds member by member.
+int batch_skip_cycle(kg_batch_s &b)
+{
+    b.ntiles_listed = -1;
+    if (b.live_pending) {
+        KG_HIP(hipEventSynchronize(b.live_ev));
+        b.live_pending = false;
+        bool ok = true;
+        for (int c = 0; c < b.nchunks && ok; ++c)
+            ok = (uint32_t)(b.live_flags[c] >> 5) == b.live_tag;
+        if (!ok) {
+            ++b.live_stale;
+        } else {
+            std::vector<int> order;
+            order.reserve((size_t)b.nchunks);
+            for (int na = BCW * BCH; na >= 0; --na) {
+                const uint32_t want = (uint32_t)na << 1 | 1u;
+                for (int c = 0; c < b.nchunks; ++c)
+                    if ((b.live_flags[c] & 31u) == want)
+                        order.push_back(c);
+            }
+            if ((int)order.size() < b.nchunks) {
+                size_t real = 0;
+                const std::vector<int4> t =
+                    deal_chunks(b.chunk_list, order, b.nx.data(), b.ny.data(), &real);
+                if (t.size() <= b.listed_cap) {
+                    std::copy(t.begin(), t.end(), b.tiles_stage);
+                    if (!t.empty())
+                        KG_HIP(hipMemcpyAsync(b.tiles_listed, b.tiles_stage,
+                                              t.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                              b.stream));
+                    b.ntiles_listed = (int)t.size();
+                }
+            }
+        }
+    }
+    b.next_tag = b.next_tag % kg::LIVE_TAGS + 1;
+    b.live_tag = b.next_tag;
+    const uint32_t *set = b.stat + (size_t)(b.steps & 1u) * b.stat_tiles;
+    hipError_t e = b.precision == KG_F64
+                       ? kg::launch_batch_live<double>(
+                             static_cast<const kg::StepArgs<double> *>(b.members), b.chunks_dev,
+                             b.nchunks, set, b.live_tag, b.live_flags, b.stream)
+                       : kg::launch_batch_live<float>(
+                             static_cast<const kg::StepArgs<float> *>(b.members), b.chunks_dev,
+                             b.nchunks, set, b.live_tag, b.live_flags, b.stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "batch liveness launch");
+    KG_HIP(hipEventRecord(b.live_ev, b.stream));
+    b.live_pending = true;
     return KG_OK;
 }
 
@@ -157,17 +256,30 @@ int launch_steps(kg_batch_s &b, int n)
     if (n > 0)
         if (int rc = ensure_status(b))
             return rc;
+    const bool lists = b.mode == KG_BATCH_LAUNCH && b.live_flags != nullptr;
+    if (!lists)
+        drop_lists(b);
     for (int s = 0; s < n; ++s) {
+        if (lists && b.steps % kg::SKIP_PERIOD == 0)
+            if (int rc = batch_skip_cycle(b))
+                return rc;
+        const bool listed = b.ntiles_listed >= 0;
+        const int4 *tiles = listed ? b.tiles_listed : b.tiles;
+        const int nt = listed ? b.ntiles_listed : b.ntiles;
         const uint32_t add = (uint32_t)(b.steps - b.base_host);
-        hipError_t e = b.precision == KG_F64
-                           ? kg::launch_step_batch<double>(
-                                 static_cast<const kg::StepArgs<double> *>(b.members), b.tiles,
-                                 b.ntiles, b.step_base, add, b.stream)
-                           : kg::launch_step_batch<float>(
-                                 static_cast<const kg::StepArgs<float> *>(b.members), b.tiles,
-                                 b.ntiles, b.step_base, add, b.stream);
+        hipError_t e = hipSuccess;
+        if (nt > 0)
+            e = b.precision == KG_F64
+                    ? kg::launch_step_batch<double>(
+                          static_cast<const kg::StepArgs<double> *>(b.members), tiles, nt,
+                          b.step_base, add, b.stream)
+                    : kg::launch_step_batch<float>(
+                          static_cast<const kg::StepArgs<float> *>(b.members), tiles, nt,
+                          b.step_base, add, b.stream);
         if (e != hipSuccess)
             return hip_fail(e, "batch step launch");
+        if (listed)
+            ++b.listed_launches;
         b.steps += 1;
     }
     return KG_OK;
@@ -213,6 +325,7 @@ int step_impl(kg_batch_s &b, int nsteps)
         }
         if (int rc = ensure_status(b))
             return rc;
+        drop_lists(b);  // the graph steps every tile
         while (nsteps >= b.graph_steps) {
             KG_HIP(hipGraphLaunch(b.exec, b.stream));
             b.steps += (uint64_t)b.graph_steps;
@@ -358,6 +471,41 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
     if (e != hipSuccess) {
         free_batch(b);
         return hip_fail(e, "batch table upload");
+    }
+    // chunk lists: state for the liveness requests and the listed tables
+    // (up to 8 tiles per chunk, every XCD padded to ceil(chunks / 8) chunks)
+    b->chunk_list = batch_chunks(nx, ny, count);
+    b->nchunks = (int)b->chunk_list.size();
+    if (b->nchunks >= kg::SKIP_MIN_CHUNKS) {
+        b->listed_cap = ((size_t)b->nchunks + BNX - 1) / BNX * BNX * BCW * BCH;
+        bool ok = hipMalloc(&b->chunks_dev, b->chunk_list.size() * sizeof(int4)) == hipSuccess;
+        if (!ok)
+            b->chunks_dev = nullptr;
+        ok = ok && hipMalloc(&b->tiles_listed, b->listed_cap * sizeof(int4)) == hipSuccess;
+        if (!ok)
+            b->tiles_listed = nullptr;
+        ok = ok && hipHostMalloc(reinterpret_cast<void **>(&b->tiles_stage),
+                                 b->listed_cap * sizeof(int4), hipHostMallocDefault) == hipSuccess;
+        if (!ok)
+            b->tiles_stage = nullptr;
+        ok = ok && hipHostMalloc(reinterpret_cast<void **>(&b->live_flags), (size_t)b->nchunks,
+                                 hipHostMallocCoherent) == hipSuccess;
+        if (!ok)
+            b->live_flags = nullptr;
+        ok = ok && hipEventCreateWithFlags(&b->live_ev, hipEventDisableTiming |
+                                                            hipEventDisableSystemFence) ==
+                       hipSuccess;
+        if (!ok) {
+            free_batch(b);
+            return fail(KG_ERR_OUT_OF_MEMORY, "allocation of the batch chunk lists failed");
+        }
+        std::memset(b->live_flags, 0, (size_t)b->nchunks);  // tag 0: never written
+        e = hipMemcpy(b->chunks_dev, b->chunk_list.data(), b->chunk_list.size() * sizeof(int4),
+                      hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            free_batch(b);
+            return hip_fail(e, "batch chunk upload");
+        }
     }
     for (int m = 0; m < count; ++m) {
         int rc = init_member(*b, m, 0, nx[m] / 2, ny[m] / 2, {}, 0.0);
@@ -661,6 +809,17 @@ int kg_batch_synchronize(kg_batch b)
         return rc;
     KG_HIP(hipSetDevice(b->device));
     KG_HIP(hipStreamSynchronize(b->stream));
+    return KG_OK;
+}
+
+int kg_batch_get_skip_stats(kg_batch b, uint64_t *listed_launches, uint64_t *stale_answers)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (!listed_launches || !stale_answers)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null output");
+    *listed_launches = b->listed_launches;
+    *stale_answers = b->live_stale;
     return KG_OK;
 }
 

@@ -97,6 +97,23 @@ struct kg_batch_s {
     int stat_tiles = 0;
     bool stat_valid = false;
     std::vector<double> theta_given;  // per member, as kg_solver_s::theta_given
+    // chunk lists (launch mode, batches of at least SKIP_MIN_CHUNKS chunks;
+    // kg_batch.hip, batch_skip_cycle): the chunks in the order the tile
+    // table deals them (member, chunk column, chunk row), the pinned flags
+    // of the request in flight, and the tile table of the listed chunks on
+    // the device with its pinned staging copy; ntiles_listed < 0: the full
+    // table
+    std::vector<int4> chunk_list;
+    int4 *chunks_dev = nullptr;
+    int nchunks = 0;
+    uint8_t *live_flags = nullptr;
+    hipEvent_t live_ev = nullptr;
+    bool live_pending = false;
+    uint32_t live_tag = 0, next_tag = 0;
+    int4 *tiles_listed = nullptr, *tiles_stage = nullptr;
+    size_t listed_cap = 0;
+    int ntiles_listed = -1;
+    uint64_t listed_launches = 0, live_stale = 0;
 
     size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }
     size_t cells(int m) const { return (size_t)nx[m] * (size_t)ny[m]; }

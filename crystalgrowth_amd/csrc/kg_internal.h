@@ -98,6 +98,12 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList
 // request that tag names).
 bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
                       std::vector<uint16_t> &out);
+// Liveness request over the chunks of a batch: flags[c] as a LiveReq writes
+// them, for chunk c = (member, chunk column, chunk row) of chunks, from the
+// status set stat_set (the input set of the next step).
+template <typename R>
+hipError_t launch_batch_live(const StepArgs<R> *members, const int4 *chunks, int nchunks,
+                             const uint32_t *stat_set, uint32_t tag, uint8_t *flags, hipStream_t s);
 // Chunk grid of the fused step for an nx x ny grid (chunks_x * chunks_y chunks).
 int fused_chunks_x(int nx);
 int fused_chunks_y(int ny);

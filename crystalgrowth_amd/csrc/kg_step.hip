@@ -932,27 +932,23 @@ __device__ __forceinline__ int live_wrap(int v, int n, int boundary)
         return v < 0 ? 0 : (v >= n ? n - 1 : v);
     return ((v % n) + n) % n;
 }
-template <typename R>
-__device__ __forceinline__ void live_request(const StepArgs<R> &a, const LiveReq &q, int tiles_x,
-                                             int tiles_y)
+// Flag byte of chunk (cx, cy) of a grid of tiles_x x tiles_y tiles (nx x ny
+// cells) from the input status words sa of its tiles and margin.
+__device__ __forceinline__ uint8_t chunk_flag(const uint32_t *__restrict__ sa, int tiles_x,
+                                              int tiles_y, int nx, int ny, int boundary, int cx,
+                                              int cy, uint32_t tag)
 {
-    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
-    const int c = blockIdx.x * NT + threadIdx.x;
-    if (c >= ncx * ncy)
-        return;
-    const int cx = c % ncx, cy = c / ncx;
     constexpr uint32_t Q = S_PHI0 | S_T0;
-    const uint32_t *__restrict__ sa = a.stat_in + a.tile_base;
     int col[LIVE_W], row[LIVE_H];
 #pragma unroll
     for (int i = 0; i < LIVE_W; ++i)
-        col[i] = live_wrap(cx * CHUNK_W - SKIP_MARGIN_X - 1 + i, tiles_x, a.boundary);
+        col[i] = live_wrap(cx * CHUNK_W - SKIP_MARGIN_X - 1 + i, tiles_x, boundary);
 #pragma unroll
     for (int j = 0; j < LIVE_H; ++j)
-        row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y - 1 + j, tiles_y, a.boundary);
-    const int wx = a.nx - (tiles_x - 1) * TX, wy = a.ny - (tiles_y - 1) * TY;
-    const bool narrow_x = a.boundary == 0 && wx < LIVE_NEED;
-    const bool narrow_y = a.boundary == 0 && wy < LIVE_NEED;
+        row[j] = live_wrap(cy * CHUNK_H - SKIP_MARGIN_Y - 1 + j, tiles_y, boundary);
+    const int wx = nx - (tiles_x - 1) * TX, wy = ny - (tiles_y - 1) * TY;
+    const bool narrow_x = boundary == 0 && wx < LIVE_NEED;
+    const bool narrow_y = boundary == 0 && wy < LIVE_NEED;
     if (!(narrow_x && col[1] == tiles_x - 1))
         col[0] = col[1];
     if (!(narrow_x && col[LIVE_W - 2] == tiles_x - 1))
@@ -972,8 +968,41 @@ __device__ __forceinline__ void live_request(const StepArgs<R> &a, const LiveReq
                 i <= SKIP_MARGIN_X + CHUNK_W)
                 act += (w & S_ACTIVE) ? 1u : 0u;
         }
-    __hip_atomic_store(q.flags + c, (uint8_t)(q.tag << 5 | act << 1 | (any ? 1u : 0u)),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return (uint8_t)(tag << 5 | act << 1 | (any ? 1u : 0u));
+}
+
+template <typename R>
+__device__ __forceinline__ void live_request(const StepArgs<R> &a, const LiveReq &q, int tiles_x,
+                                             int tiles_y)
+{
+    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
+    const int c = blockIdx.x * NT + threadIdx.x;
+    if (c >= ncx * ncy)
+        return;
+    const uint8_t f = chunk_flag(a.stat_in + a.tile_base, tiles_x, tiles_y, a.nx, a.ny, a.boundary,
+                                 c % ncx, c / ncx, q.tag);
+    __hip_atomic_store(q.flags + c, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// The same request over the chunks of a batch (chunks[c] = member, chunk
+// column, chunk row): the member's input status set is stat_set + its
+// tile_base.  A member without status has every chunk live and heavy.
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_live(const StepArgs<R> *__restrict__ members,
+                                                    const int4 *__restrict__ chunks, int nchunks,
+                                                    const uint32_t *__restrict__ stat_set,
+                                                    uint32_t tag, uint8_t *__restrict__ flags)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= nchunks)
+        return;
+    const int4 d = chunks[c];
+    const StepArgs<R> &a = members[d.x];
+    uint8_t f = (uint8_t)(tag << 5 | (uint32_t)(CHUNK_W * CHUNK_H) << 1 | 1u);
+    if (a.stat_in != nullptr)
+        f = chunk_flag(stat_set + a.tile_base, a.tiles_x, a.tiles_y, a.nx, a.ny, a.boundary, d.y,
+                       d.z, tag);
+    __hip_atomic_store(flags + c, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 template <typename R, int BC>
@@ -1385,6 +1414,17 @@ bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
     return true;
 }
 
+template <typename R>
+hipError_t launch_batch_live(const StepArgs<R> *members, const int4 *chunks, int nchunks,
+                             const uint32_t *stat_set, uint32_t tag, uint8_t *flags, hipStream_t s)
+{
+    if (nchunks <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL((k_batch_live<R>), dim3((nchunks + 255) / 256), dim3(256), 0, s, members,
+                       chunks, nchunks, stat_set, tag, flags);
+    return hipGetLastError();
+}
+
 int fused_chunks_x(int nx) { return ((nx + TX - 1) / TX + CHUNK_W - 1) / CHUNK_W; }
 int fused_chunks_y(int ny) { return ((ny + TY - 1) / TY + CHUNK_H - 1) / CHUNK_H; }
 
@@ -1545,6 +1585,8 @@ hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double 
 #define KG_INSTANTIATE(R)                                                                        \
     template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t, const LiveList *, \
                                              const LiveReq *);                                \
+    template hipError_t launch_batch_live<R>(const StepArgs<R> *, const int4 *, int,             \
+                                             const uint32_t *, uint32_t, uint8_t *, hipStream_t); \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
                                              const uint32_t *, uint32_t, hipStream_t);           \
     template hipError_t launch_step_batch_direct<R>(const StepArgs<R> *, const int4 *, int,      \

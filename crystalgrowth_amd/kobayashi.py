@@ -383,6 +383,13 @@ class Batch:
     def synchronize(self) -> None:
         check(lib().kg_batch_synchronize(self._h))
 
+    def skip_stats(self) -> dict:
+        """Launches that covered only the live chunks, and liveness answers
+        dropped as not yet visible (kg_batch_get_skip_stats)."""
+        listed, stale = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        check(lib().kg_batch_get_skip_stats(self._h, ctypes.byref(listed), ctypes.byref(stale)))
+        return {"listed_launches": listed.value, "stale_answers": stale.value}
+
     @property
     def step_count(self) -> int:
         v = ctypes.c_uint64()

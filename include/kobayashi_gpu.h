@@ -235,6 +235,9 @@ int kg_batch_step(kg_batch b, int nsteps);
 int kg_batch_step_timed(kg_batch b, int nsteps, double *ms);
 int kg_batch_synchronize(kg_batch b);
 int kg_batch_get_step_count(kg_batch b, uint64_t *steps);
+/* Diagnostic, no counterpart in the reference: as kg_get_skip_stats, for a
+ * batch in KG_BATCH_LAUNCH mode with at least 256 chunks over its members. */
+int kg_batch_get_skip_stats(kg_batch b, uint64_t *listed_launches, uint64_t *stale_answers);
 /* kg_adjoint for every member at once: targets[m] and d_T0[m] (optional;
  * the array or an entry may be NULL) are host fields of member m; out[m]
  * receives member m's loss and gradients.  One launch per kernel covers all

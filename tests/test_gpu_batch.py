@@ -161,3 +161,43 @@ def test_member_with_wide_stored_angles(kb):
     ref = plain(kb, nx, ny, "f32", p, (phi, T, th), 20)
     for x, y in zip(got, ref):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_chunk_lists_match_plain_solver_bitwise(kb, prec):
+    """Launch mode over a batch of 256+ chunks takes chunk lists (members'
+    far fields left out, heaviest chunks first): every member still evolves
+    bit for bit like a dense plain handle.  Members differ in size, boundary
+    and partial edge tiles; nuclei sit off centre so fronts cross members'
+    chunk edges and, on periodic members, wrap."""
+    sizes = [(1024, 960), (770, 610), (1030, 901), (640, 480), (900, 905)]
+    params = [
+        kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1, seed=1, instance=0),
+        kb.default_params(delta=0.02, aniso=4.0, noise_a=0.02, boundary=0, seed=2, instance=1),
+        kb.default_params(delta=0.05, noise_a=0.01, boundary=0, seed=3, instance=2),
+        kb.default_params(aniso=5.0, theta0=0.7, noise_a=0.01, boundary=1, seed=4, instance=3),
+        kb.default_params(delta=0.06, K=0.9, noise_a=0.01, boundary=0, seed=5, instance=4),
+    ]
+    spots = [(100, 120), (766, 3), (5, 895), (320, 240), (899, 450)]
+    marks = [1, 9, 40, 151, 400]
+    with kb.Batch(sizes, params, prec) as b:
+        b.set_mode("launch")
+        for m, (cx, cy) in enumerate(spots):
+            b.reset_disk(m, cx, cy, 6.0)
+        got, done = {}, 0
+        for mk in marks:
+            b.step(mk - done)
+            done = mk
+            got[mk] = [b.fields(m) for m in range(len(sizes))]
+        stats = b.skip_stats()
+    for m, ((nx, ny), p, (cx, cy)) in enumerate(zip(sizes, params, spots)):
+        with kb.Solver(nx, ny, prec, p) as s:
+            s.set_kernel("fused_dense")
+            s.reset_disk(cx, cy, 6.0)
+            done = 0
+            for mk in marks:
+                s.step(mk - done)
+                done = mk
+                for name, x, y in zip(("phi", "T", "theta"), got[mk][m], s.fields()):
+                    assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (m, mk, name)
+    assert stats["listed_launches"] >= 350 and stats["stale_answers"] == 0, stats
