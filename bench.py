@@ -1,34 +1,44 @@
 """Benchmark of the Kobayashi step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--kernel fused|two_pass]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--start S] [--size N]
 
 Workload (the single-large-grid configuration): 4096 x 4096 FP32 grid,
 zero-flux boundaries, dx = 0.03, dt = 1e-4, tau = 3e-4, eps_bar = 0.01,
 delta = 0.04, j = 6, theta0 = 0.2, alpha = 0.9, gamma = 10, K = 2.0, Teq = 1,
-noise a = 0.01, nine r = 5 nuclei at Philox-drawn centres (seed 0, margin 8).
-A step is one explicit-Euler step of the whole grid; the timed region is K
-steps of that trajectory starting from its initial state (default K = 20000,
-the configuration's full horizon), with the fields resident in HBM.
+noise a = 0.01, nine r = 5 nuclei at Philox-drawn centres (seed 0, margin 8),
+a 20000-step horizon.
 
-Prints one JSON line.  `value` is cell-updates per second of the timed
-region; `roofline` prices the step's average duration on the GPU (HIP events
-on the solver's own stream around the timed steps: each step is one launch
-of the fused kernel, over every 2 x 4-tile chunk or over the live ones; every
-7th launch also answers the host's liveness request) at 16 algorithmic bytes per
-cell-update against the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the reference's
-arithmetic restated in C, one thread) on a bounded sample of the same
-workload (`cpu_all_cores`: the same oracle on OpenMP threads, labelled
-apart).  Beside the headline it reports:
-  batched   -- the 512-member ensemble (sides 128..1024, per-member
-               parameters) over its 5000-step horizon from disk nuclei: one
-               batched launch per step (autotuned launch mode) vs looping the
-               plain solver per member (one handle and stream per member)
-  adjoint   -- loss + gradients of 1000 steps on 512^2 (checkpoint every 50)
-               vs the plain forward of the same steps
-  dense_kernel -- the same 4096^2 trajectory on the fused launch without tile
-               status (KG_KERNEL_FUSED_DENSE, bitwise the same results)
+The headline step is one explicit-Euler step of the whole grid in which every
+cell is loaded, updated and stored: the fused kernel without tile status
+(k_step_dense).  The state is first advanced, untimed, to step S of the
+trajectory (default 10000, mid-horizon: growing dendrites, the front, and the
+quiescent far field together), then W untimed and K timed dense steps follow,
+with the fields resident in HBM.  `value` is cell-updates per second over the
+K timed steps (wall clock between two synchronisations); `roofline` prices
+the average duration of the dense launch (HIP events on the solver's own
+stream, one launch per step) at 16 algorithmic bytes per cell-update
+(phi and T read once and written once) against the 8 TB/s HBM peak.
+`cpu_baseline` times the CPU oracle (the reference's arithmetic restated in
+C, one thread) on a bounded sample of the same workload.
+
+Beside the headline, labelled apart:
+  trajectory   -- the whole 20000-step horizon from the initial state, with
+                  the default kernel (tile status: exact skips of work the
+                  status proves unneeded, bitwise the dense results) and
+                  with the dense kernel
+  batched      -- the 512-member ensemble (sides 128..1024, per-member
+                  parameters) over its 5000-step horizon from disk nuclei: one
+                  batched launch per step (autotuned launch mode) vs looping the
+                  plain solver per member (one handle and stream per member)
+  adjoint      -- loss + gradients of 1000 steps on 512^2 (checkpoint every 50)
+                  vs the plain forward of the same steps
+  fit          -- the 64-member batched parameter fit (500 steps, 50 Adam
+                  iterations): recovered-parameter errors, and one batched
+                  gradient against the same gradient looped over plain handles
   reference_config -- the reference's configuration (300^2 FP64, 2000 steps)
-               on the GPU and on the CPU oracle, compared cell by cell
+                  on the GPU and on the CPU oracle, compared cell by cell
+  cpu_all_cores -- the oracle on OpenMP threads
+--headline-only skips everything but the headline (for profiler runs);
 --calibrate-copy runs only a device copy of known size (for scaling the
 rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes).
 """
@@ -57,24 +67,26 @@ SEEDS = dict(count=9, r=5.0, margin=8, seed=0)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=200, help="timed dense steps (K)")
+    ap.add_argument("--warmup", type=int, default=20, help="untimed dense steps before them (W)")
+    ap.add_argument("--start", type=int, default=10000,
+                    help="trajectory step the dense window starts from (S; reached untimed)")
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--kernel", default="fused", choices=["fused", "fused_dense", "two_pass"])
     ap.add_argument("--cpu-steps", type=int, default=10,
                     help="steps of the CPU oracle sample (0 skips the CPU baseline)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline window (for profiler runs)")
     ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--horizon", type=int, default=20000,
+                    help="steps of the trajectory legs (0 skips them)")
     ap.add_argument("--batch-steps", type=int, default=5000,
                     help="steps of the batched-vs-looped comparison (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--fit-iterations", type=int, default=50,
+                    help="Adam iterations of the batched fit (0 skips it)")
     ap.add_argument("--no-looped", action="store_true",
                     help="batched leg without the looped plain handles (for profiling)")
-    ap.add_argument("--no-dense", action="store_true",
-                    help="skip the same trajectory on the fused kernel without tile status")
     ap.add_argument("--calibrate-copy", action="store_true")
-    ap.add_argument("--sync-every", type=int, default=0,
-                    help="synchronise after every N timed steps (for counter-collection "
-                         "profiler runs; 0 = queue all steps)")
     return ap.parse_args()
 
 
@@ -98,10 +110,11 @@ def cpu_model() -> str:
 
 
 def traffic_from_profiles(size: int):
-    """HBM bytes per launch of the fused step measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration) and
-    recorded under profiles/; None when no measurement exists."""
-    path = os.path.join(ROOT, "profiles", f"kobayashi_step_{size}x{size}_fp32_traffic.json")
+    """HBM bytes per launch of the dense step measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE of k_step_dense over the headline window, scaled
+    by the copy-kernel calibration) and recorded under profiles/; None when no
+    measurement exists for this size."""
+    path = os.path.join(ROOT, "profiles", f"k_step_dense_{size}x{size}_fp32_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
@@ -110,19 +123,21 @@ def traffic_from_profiles(size: int):
         return None
 
 
-def cpu_baseline(size: int, nsteps: int):
+def cpu_baseline(state, nsteps: int, start: int):
+    """The CPU oracle (one thread) for nsteps steps from the state the GPU
+    reached at trajectory step `start` (the headline window's first state)."""
     from oracle import oracle as O
     p = O.default_params(**WORKLOAD)
-    s, _ = O.init_seeds(size, size, SEEDS["count"], SEEDS["r"], SEEDS["margin"], SEEDS["seed"],
-                        dtype=np.float32)
+    size = state.phi.shape[0]
     t0 = time.perf_counter()
-    O.run(s, p, nsteps)
+    O.run(state, p, nsteps, step0=start)
     dt = time.perf_counter() - t0
-    cups = size * size * nsteps / dt
+    cups = state.phi.size * nsteps / dt
     return {"value": cups / 1e9, "unit": "Gcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"CPU oracle (reference float arithmetic restated in C, gcc -O2, 1 thread) "
-                      f"on the {size}x{size} workload, steps 0-{nsteps - 1} from the seeded "
-                      f"initial state, {dt:.1f} s on '{cpu_model()}' ({os.cpu_count()} cores)"}
+                      f"on the {size}x{size} workload, steps {start}-{start + nsteps - 1} from the "
+                      f"GPU's state at step {start}, {dt:.1f} s on '{cpu_model()}' "
+                      f"({os.cpu_count()} cores)"}
 
 
 def cpu_all_cores(size: int, nsteps: int):
@@ -181,21 +196,31 @@ def reference_config(kb, cpu_steps: int):
             "solid_fraction": float((gphi > 0.5).mean())}
 
 
-def dense_run(kb, size: int, nsteps: int):
-    """The same trajectory with KG_KERNEL_FUSED_DENSE: the fused launch with
-    no tile status (every tile loads and stores all of its fields; bitwise
-    the same results, tests/test_gpu_tile_status.py).  Shows how much of the
-    headline comes from skipping work the status proves unnecessary."""
-    with kb.Solver(size, size, "f32", kb.default_params(**WORKLOAD)) as s:
-        s.set_kernel("fused_dense")
-        s.reset_seeds(**SEEDS)
-        s.synchronize()
-        ms = s.step_timed(nsteps)
-    per = ms / nsteps / 1e3
-    ach = ALG_BYTES_FP32 * size * size / per / 1e9
-    return {"value": round(size * size / per / 1e9, 3), "unit": "Gcell-updates/s",
-            "avg_launch_us": round(per * 1e6, 3), "achieved_GBs": round(ach, 1),
-            "frac": round(ach / HBM_PEAK_GBS, 4)}
+def trajectory(kb, size: int, nsteps: int):
+    """The whole horizon from the initial state, timed with HIP events: the
+    default kernel (tile status and launches of the live chunks only, exact:
+    bitwise the dense results, tests/test_gpu_tile_status.py) and the dense
+    kernel.  Labelled apart from the headline: the default kernel skips
+    loads, stores and whole chunks the status proves unneeded, so its rate is
+    not priced against the roofline."""
+    out = {"steps": nsteps}
+    for name in ("fused", "fused_dense"):
+        with kb.Solver(size, size, "f32", kb.default_params(**WORKLOAD)) as s:
+            s.set_kernel(name)
+            s.reset_seeds(**SEEDS)
+            s.synchronize()
+            ms = s.step_timed(nsteps)
+            st = s.stats()
+        per = ms / nsteps / 1e3
+        key = "exact_skip" if name == "fused" else "dense"
+        out[key] = {"Gcell_updates_per_s": round(size * size / per / 1e9, 3),
+                    "avg_step_us": round(per * 1e6, 3),
+                    "final_solid_fraction": st["solid_fraction"]}
+    out["dense"]["alg_frac_of_hbm_peak"] = round(
+        ALG_BYTES_FP32 * size * size / (out["dense"]["avg_step_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+    out["same_final_state"] = (out["exact_skip"]["final_solid_fraction"] ==
+                               out["dense"]["final_solid_fraction"])
+    return out
 
 
 def fp64_error(kb, size: int, nsteps: int = 100):
@@ -303,6 +328,25 @@ def adjoint_vs_forward(kb):
             "note": "gradients checked against central differences in tests/test_gpu_adjoint.py"}
 
 
+def fit_summary(kb, iterations: int):
+    """The 64-member batched parameter fit and one gradient batched vs looped."""
+    from crystalgrowth_amd import configs, fit
+    c = configs.C4
+    sizes, hidden, guess = configs.c4_members(c["count"])
+    t0 = time.perf_counter()
+    res = fit.fit(sizes, hidden, guess, c["steps"], iterations, c["every"])
+    tf = time.perf_counter() - t0
+    targets = fit.make_targets(sizes, hidden, c["steps"], "f32")
+    tb = fit.batched_adjoint_seconds(sizes, guess, targets, c["steps"], c["every"])
+    tl = fit.looped_adjoint_seconds(sizes, guess, targets, c["steps"], c["every"])
+    return {"members": len(sizes), "steps": c["steps"], "iterations": iterations,
+            "fit_s": round(tf, 2), "loss_first": res["loss_first"], "loss_last": res["loss_last"],
+            "param_error_start": res["param_error_start"],
+            "param_error_end": res["param_error_end"],
+            "one_gradient_batched_s": round(tb, 4), "one_gradient_looped_s": round(tl, 4),
+            "batched_speedup_vs_looped": round(tl / tb, 3)}
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -320,34 +364,36 @@ def main():
 
     n = args.size
     cells = n * n
+    first = args.start + args.warmup
     with kb.Solver(n, n, "f32", kb.default_params(**WORKLOAD)) as s:
-        s.set_kernel(args.kernel)
-        # warmup on the same workload, then restart the trajectory at step 0
+        # reach step S of the trajectory untimed (default kernel: bitwise the
+        # dense results), then W untimed and K timed dense steps
         s.reset_seeds(**SEEDS)
+        s.step(args.start)
+        s.synchronize()
+        start_state = None
+        if args.cpu_steps > 0 and not args.headline_only:
+            from oracle import oracle as O
+            start_state = O.State(*s.fields())
+        s.set_kernel("fused_dense")
         if args.warmup > 0:
             s.step(args.warmup)
         s.synchronize()
-        s.reset_seeds(**SEEDS)
         torch_sync()
-        s.synchronize()
         t0 = time.perf_counter()
-        if args.sync_every > 0:
-            ev_ms, done = 0.0, 0
-            while done < args.steps:
-                k = min(args.sync_every, args.steps - done)
-                ev_ms += s.step_timed(k)
-                done += k
-        else:
-            ev_ms = s.step_timed(args.steps)  # HIP events on the solver's stream
+        ev_ms = s.step_timed(args.steps)  # HIP events on the solver's stream
         s.synchronize()
         torch_sync()
         wall = time.perf_counter() - t0
         st = s.stats()
 
-    ms_per_step = ev_ms / max(args.steps, 1)
-    value = cells * args.steps / (wall) / 1e9
-    launch_s = ms_per_step / 1e3
-    achieved = ALG_BYTES_FP32 * cells / launch_s / 1e9 if launch_s > 0 else 0.0
+    per_launch = ev_ms / 1e3 / max(args.steps, 1)
+    value = cells * args.steps / wall / 1e9
+    achieved = ALG_BYTES_FP32 * cells / per_launch / 1e9 if per_launch > 0 else 0.0
+    frac = achieved / HBM_PEAK_GBS
+    if frac > 1.0:
+        print(f"bench: roofline fraction {frac:.3f} above 1 -- the timed launches did not move "
+              f"the algorithmic bytes", file=sys.stderr, flush=True)
     rec = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -361,31 +407,39 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"kobayashi-{n}x{n}-fp32-zero-flux-noise-9-seeds",
-                   "nx": n, "ny": n, "kernel": args.kernel, **WORKLOAD,
+        "config": {"workload": f"kobayashi-{n}x{n}-fp32-zero-flux-noise-9-seeds-dense-steps-"
+                               f"{first}-{first + args.steps - 1}",
+                   "nx": n, "ny": n, "kernel": "k_step_dense (every cell loaded, updated, stored)",
+                   "window_start_step": first, "horizon": 20000, **WORKLOAD,
                    "nuclei": SEEDS["count"], "nucleus_radius": SEEDS["r"],
                    "dx": 0.03, "dt": 1e-4, "tau": 3e-4, "eps_bar": 0.01,
                    "alpha": 0.9, "gamma": 10.0, "teq": 1.0},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "unit": "GB/s", "frac": round(frac, 4),
                      "traffic": traffic_from_profiles(n),
-                     "kernel": ("k_pass1+k_pass2" if args.kernel == "two_pass" else
-                                "k_step_fused<float> (all chunks) or k_step_fused_listed<float> "
-                                "(live chunks, heaviest first; every 7th launch also answers the liveness request)"),
-                     "avg_launch_us": round(ms_per_step * 1e3, 3),
-                     "alg_bytes_per_cell_update": ALG_BYTES_FP32},
-        "final_state": {"solid_fraction": st["solid_fraction"], "sum_phi": st["sum_phi"],
-                        "nonfinite": st["nonfinite"]},
+                     "kernel": "k_step_dense<float, zero-flux>: one launch per step, every tile",
+                     "avg_launch_us": round(per_launch * 1e6, 3),
+                     "alg_bytes_per_cell_update": ALG_BYTES_FP32,
+                     "alg_bytes_per_launch": ALG_BYTES_FP32 * cells,
+                     "frac_at_most_1": frac <= 1.0},
+        "final_state": {"step": first + args.steps, "solid_fraction": st["solid_fraction"],
+                        "sum_phi": st["sum_phi"], "nonfinite": st["nonfinite"]},
     }
-    if args.kernel == "fused" and not args.no_dense:
-        rec["dense_kernel"] = dense_run(kb, n, args.steps)
+    if args.headline_only:
+        print(json.dumps(rec), flush=True)
+        return
+    if args.horizon > 0:
+        rec["trajectory"] = trajectory(kb, n, args.horizon)
     if not args.no_fp64_check:
         rec["fp64_rel_err_100_steps"] = fp64_error(kb, n)
     if args.batch_steps > 0:
         rec["batched"] = batched_vs_looped(kb, args.batch_steps, not args.no_looped)
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_vs_forward(kb)
-    rec["cpu_baseline"] = cpu_baseline(n, args.cpu_steps) if args.cpu_steps > 0 else None
+    if args.fit_iterations > 0:
+        rec["fit"] = fit_summary(kb, args.fit_iterations)
+    rec["cpu_baseline"] = (cpu_baseline(start_state, args.cpu_steps, args.start)
+                           if start_state is not None else None)
     if args.cpu_steps > 0:
         rec["cpu_all_cores"] = cpu_all_cores(n, 6 * args.cpu_steps)
         rec["reference_config"] = reference_config(kb, 20 * args.cpu_steps)

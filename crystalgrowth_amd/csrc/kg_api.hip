@@ -723,34 +723,56 @@ int kg_reduce(kg_handle h, kg_stats *out)
     return KG_OK;
 }
 
-int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order)
+}  // extern "C"
+
+namespace {
+
+// kg_colormap / kg_colormap_rgba8: the ramp on the device into a scratch
+// buffer, then one copy to the caller's buffer
+int colormap_impl(kg_handle h, void *dst, size_t bytes, int order, bool rgba8)
 {
     if (int rc = check_handle(h))
         return rc;
-    if (!rgba)
+    if (!dst)
         return fail(KG_ERR_INVALID_ARGUMENT, "null rgba");
     if (order != 0 && order != 1)
         return fail(KG_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
     if (order == 1 && h->nx != h->ny)
         return fail(KG_ERR_INVALID_ARGUMENT, "the reference's buffer order needs a square grid");
-    if (bytes != h->cells() * 4 * sizeof(float))
-        return fail(KG_ERR_SIZE, "rgba buffer must hold 4 floats per cell");
+    const size_t per_cell = rgba8 ? 4 : 4 * sizeof(float);
+    if (bytes != h->cells() * per_cell)
+        return fail(KG_ERR_SIZE, rgba8 ? "rgba buffer must hold 4 bytes per cell"
+                                       : "rgba buffer must hold 4 floats per cell");
     KG_HIP(hipSetDevice(h->device));
-    float *d = nullptr;
+    void *d = nullptr;
     KG_HIP(hipMalloc(&d, bytes));
     hipError_t e = h->precision == KG_F64
                        ? kg::launch_colormap<double>(static_cast<const double *>(h->phi[h->cur]),
-                                                     h->nx, h->ny, order, d, h->stream)
+                                                     h->nx, h->ny, order, d, rgba8, h->stream)
                        : kg::launch_colormap<float>(static_cast<const float *>(h->phi[h->cur]),
-                                                    h->nx, h->ny, order, d, h->stream);
+                                                    h->nx, h->ny, order, d, rgba8, h->stream);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(rgba, d, bytes, hipMemcpyDeviceToHost, h->stream);
+        e = hipMemcpyAsync(dst, d, bytes, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess)
         e = hipStreamSynchronize(h->stream);
     (void)hipFree(d);
     if (e != hipSuccess)
         return hip_fail(e, "colormap");
     return KG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order)
+{
+    return colormap_impl(h, rgba, bytes, order, false);
+}
+
+int kg_colormap_rgba8(kg_handle h, uint8_t *rgba, size_t bytes, int order)
+{
+    return colormap_impl(h, rgba, bytes, order, true);
 }
 
 int kg_front_profile(kg_handle h, double cx, double cy, int n, double level, double *radii)

@@ -135,8 +135,10 @@ template <typename R>
 hipError_t launch_reduce(const R *phi, const R *T, size_t n, double K, double *partials,
                          ReduceOut *out, hipStream_t s);
 
+// phi -> colour ramp: 4 floats per cell (rgba8 false) or 4 bytes (true)
 template <typename R>
-hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba, hipStream_t s);
+hipError_t launch_colormap(const R *phi, int nx, int ny, int order, void *rgba, bool rgba8,
+                           hipStream_t s);
 
 // Front distances along n rays from (cx, cy) to phi = level (FP64 results,
 // `radii` device memory of n doubles).

@@ -64,6 +64,18 @@ template <> struct Consts<double> {
 __device__ __forceinline__ bool is_pos_zero(float v) { return __float_as_uint(v) == 0u; }
 __device__ __forceinline__ bool is_pos_zero(double v) { return __double_as_longlong(v) == 0ll; }
 
+// a * b rounded to the type and never fused into a neighbouring add: the
+// empty asm takes the product as an opaque register value (the compiler's
+// contraction of a * b - c into an fma would otherwise be free to differ
+// between unrolled copies of the same expression, and with it the bits)
+template <typename R>
+__device__ __forceinline__ R mul_rn(R a, R b)
+{
+    R p = a * b;
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
 // 4 consecutive elements, 16-byte aligned (float) or 2 x 16 bytes (double)
 __device__ __forceinline__ void load4(const float *p, float v[4])
 {
@@ -280,16 +292,19 @@ __device__ __forceinline__ int map_tile(int t, int n)
 }
 
 // LDS of one tile: phi (halo 2), T (halo 1) and the pass-1 values that pass 2
-// reads from neighbours, each row covering columns x0-4 .. x0+67.  The old
-// theta of non-flat tiles is staged in B: every pass-1 cell reads its own
-// theta slot before it writes its own B there.
+// reads from neighbours -- eps^2 and the factor F = eps eps' of the cross
+// terms -- each row covering columns x0-4 .. x0+67.  The cross terms need F
+// times a neighbour's gx or gy; pass 2 recomputes those gradients from the
+// staged phi with the operands and operations pass 1 used, so they round
+// alike and only F is kept (37.5 KB per FP32 tile: 4 blocks of 8 waves fit a
+// CU's 160 KB).  The old theta of non-flat tiles is staged in F: every pass-1
+// cell reads its own theta slot before it writes its own F there.
 template <typename R>
 struct TileSmem {
     R P[PH * LW];
     R T[QH * LW];
     R E2[QH * LW];
-    R A[QH * LW];
-    R B[QH * LW];
+    R F[QH * LW];
     R Z[2];           // eps, eps eps' at theta = +0
     int W[NT / 64];   // per-wave flatness bits
     int W2[NT / 64];  // per-wave status bits of the outputs
@@ -443,38 +458,37 @@ struct ZeroAngle {
 };
 
 // pass-1 values of one cell from its phi neighbourhood (rows m, z, p of sP
-// around column c) and its old theta.  Returns e^2, eps eps' gx, eps eps' gy
-// and the theta the step leaves (fired: true when a branch rewrote it).
+// around column c) and its old theta.  Returns e^2, F = eps eps' and the
+// theta the step leaves (fired: true when a branch rewrote it).
 template <typename R>
 __device__ __forceinline__ void pass1_cell(const StepArgs<R> &a, const ZeroAngle<R> &z0,
                                            const R *m, const R *z, const R *p, int c,
-                                           R th_old, R &e2, R &A, R &B, R &th, bool &fired)
+                                           R th_old, R &e2, R &F, R &th, bool &fired)
 {
     const R gx = (z[c + 1] - z[c - 1]) * a.inv_dx;
     const R gy = (p[c] - m[c]) * a.inv_dy;
     th = th_old;  // kept unless a branch fires
     fired = theta_branch(gx, gy, th);
-    R eps, f;
+    R eps;
     if (is_pos_zero(th)) {
         eps = z0.eps;
-        f = z0.f;
+        F = z0.f;
     } else {
-        eps_of_angle(a, th, eps, f);
+        eps_of_angle(a, th, eps, F);
     }
     e2 = eps * eps;
-    A = f * gx;
-    B = f * gy;
 }
 
-// One explicit-Euler step of the 64 x 30 output tile at (x0, y0).
-template <typename R, int BC>
+// One explicit-Euler step of the 64 x 30 output tile at (x0, y0).  TRACK
+// false: the caller passes no tile status (the dense launch), and the status
+// code is not compiled in.
+template <typename R, int BC, bool TRACK = true>
 __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, TileSmem<R> &sm)
 {
     R *__restrict__ sP = sm.P;
     R *__restrict__ sT = sm.T;
     R *__restrict__ sE2 = sm.E2;
-    R *__restrict__ sA = sm.A;
-    R *__restrict__ sB = sm.B;
+    R *__restrict__ sF = sm.F;
     const int nx = a.nx, ny = a.ny;
     const int tid = threadIdx.x;
 
@@ -483,7 +497,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     // this one (so on the tile and all of its halos); has: bits that already
     // hold on this tile of the output buffers; active: the tile took the full
     // path last step, so theta is staged with phi and T.
-    const bool track = a.stat_in != nullptr;
+    const bool track = TRACK && a.stat_in != nullptr;
     uint32_t known = 0, has = 0;
     bool active = false;
     int sidx = 0;
@@ -573,7 +587,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         if (spec_T || !T_known)
             commit_rows<R, QH, 2>(gt, sT, finite, finite);
         if (active)
-            commit_rows<R, QH, 0>(gh, sB, unused0, unused1);
+            commit_rows<R, QH, 0>(gh, sF, unused0, unused1);
     } else {
         if (!phi_known)
             stage_rows<R, BC, 1>(a.phi_in, sP, nx, ny, x0, y0 - 2, PH, all0, all1);
@@ -581,7 +595,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             stage_rows<R, BC, 2>(a.T_in, sT, nx, ny, x0, y0 - 1, QH, finite, finite);
         if (active) {
             int unused0 = 1, unused1 = 1;
-            stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+            stage_rows<R, BC, 0>(a.theta, sF, nx, ny, x0, y0 - 1, QH, unused0, unused1);
         }
     }
     if (T_known && !spec_T)
@@ -603,15 +617,15 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     const bool flat = fl != 0;
 
     if (!flat) {
-        // ---- stage the old theta of the pass-1 cells into B ---------------
+        // ---- stage the old theta of the pass-1 cells into F ---------------
         if (!active) {
             int unused0 = 1, unused1 = 1;
             if ((nx & 3) == 0) {
                 Seg2<R> gh;
                 issue_rows_w<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
-                commit_rows<R, QH, 0>(gh, sB, unused0, unused1);
+                commit_rows<R, QH, 0>(gh, sF, unused0, unused1);
             } else {
-                stage_rows<R, BC, 0>(a.theta, sB, nx, ny, x0, y0 - 1, QH, unused0, unused1);
+                stage_rows<R, BC, 0>(a.theta, sF, nx, ny, x0, y0 - 1, QH, unused0, unused1);
             }
         }
         if (phi_known)
@@ -630,9 +644,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             const int g = tid % NG, r = tid / NG;       // pass-1 row r: global y0-1+r
             const int kc = 4 * g + 4;                   // LDS column of the first cell
             const int ux = x0 + 4 * g, uy = y0 - 1 + r;
-            R e2[4], A[4], B[4], th[4], thold[4];
+            R e2[4], F[4], th[4], thold[4];
             bool fired[4];
-            load4(&sB[r * LW + kc], thold);
+            load4(&sF[r * LW + kc], thold);
             // zero-flux ghost rows/columns take the values of the cell they
             // clamp to; periodic halos already hold wrapped values
             const bool shifted = BC == 1 && (uy < 0 || uy >= ny || ux + 3 >= nx);
@@ -647,7 +661,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             if (!shifted) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
-                    pass1_cell(a, z0, m, z, p, e + 1, thold[e], e2[e], A[e], B[e], th[e], fired[e]);
+                    pass1_cell(a, z0, m, z, p, e + 1, thold[e], e2[e], F[e], th[e], fired[e]);
             } else {
                 const int cy = uy < 0 ? 0 : (uy >= ny ? ny - 1 : uy);
                 const int lr = r + (cy - uy);
@@ -656,12 +670,11 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                     const int cx = ux + e >= nx ? nx - 1 : ux + e;
                     const int lc = kc + e + (cx - (ux + e));
                     pass1_cell(a, z0, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
-                               thold[e], e2[e], A[e], B[e], th[e], fired[e]);
+                               thold[e], e2[e], F[e], th[e], fired[e]);
                 }
             }
             store4(&sE2[r * LW + kc], e2);
-            store4(&sA[r * LW + kc], A);
-            store4(&sB[r * LW + kc], B);
+            store4(&sF[r * LW + kc], F);
             // theta of owned cells: rewrite the group (kept values unchanged)
             // when any of its cells fired
             if (r >= 1 && r <= TY && uy < ny && (fired[0] | fired[1] | fired[2] | fired[3])) {
@@ -688,13 +701,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 lc += cx - ux;
                 lr += cy - uy;
             }
-            R e2, A, B, th;
+            R e2, F, th;
             bool fired;
             pass1_cell(a, z0, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
-                       sB[r * LW + lcol], e2, A, B, th, fired);
+                       sF[r * LW + lcol], e2, F, th, fired);
             sE2[r * LW + lcol] = e2;
-            sA[r * LW + lcol] = A;
-            sB[r * LW + lcol] = B;
+            sF[r * LW + lcol] = F;
         }
         __syncthreads();
     }
@@ -726,7 +738,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         }
     } else {
         // in phases, so that each set of LDS windows dies before the next is
-        // read (register pressure): T, phi, eps^2, then the cross terms
+        // read (register pressure): T, phi, the cross terms, then eps^2
         R lapT[4], Tc[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -736,7 +748,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         phase_end(lapT);
         phase_end(Tc);
         phase_fence();
-        R phi[4], lap[4], gx[4], gy[4];
+        // phi, its Laplacian and gradient, and the neighbours' gradients the
+        // cross terms take: gx at N and S, gy at the window's 6 columns (the
+        // pass-1 expressions of those cells on the same staged values)
+        R phi[4], lap[4], gx[4], gy[4], gxN[4], gxS[4], gyW, gyE;
         {
             R Pm[6], Pz[6], Pp[6];
             win6(&sP[(rw + 1) * LW], kc, Pm);
@@ -749,12 +764,36 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 lap[e] = lap9(Pm, Pz, Pp, c) * a.inv_3dx2;
                 gx[e] = (Pz[c + 1] - Pz[c - 1]) * a.inv_dx;
                 gy[e] = (Pp[c] - Pm[c]) * a.inv_dy;
+                gxN[e] = (Pp[c + 1] - Pp[c - 1]) * a.inv_dx;
+                gxS[e] = (Pm[c + 1] - Pm[c - 1]) * a.inv_dx;
             }
+            gyW = (Pp[0] - Pm[0]) * a.inv_dy;
+            gyE = (Pp[5] - Pm[5]) * a.inv_dy;
         }
         phase_end(phi);
         phase_end(lap);
         phase_end(gx);
         phase_end(gy);
+        phase_end(gxN);
+        phase_end(gxS);
+        phase_fence();
+        // t1 = (F_N gx_N - F_S gx_S) / dy, t2 = -(F_E gy_E - F_W gy_W) / dx
+        // (:197-203), each product rounded as the reference rounds it
+        R t1[4], t2[4];
+        {
+            R Fm[4], Fz[6], Fp[4];
+            load4(&sF[rw * LW + kc], Fm);
+            win6(&sF[(rw + 1) * LW], kc, Fz);
+            load4(&sF[(rw + 2) * LW + kc], Fp);
+            const R gy6[6] = {gyW, gy[0], gy[1], gy[2], gy[3], gyE};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                t1[e] = (mul_rn(Fp[e], gxN[e]) - mul_rn(Fm[e], gxS[e])) * a.inv_dy;
+                t2[e] = -(mul_rn(Fz[e + 2], gy6[e + 2]) - mul_rn(Fz[e], gy6[e])) * a.inv_dx;
+            }
+        }
+        phase_end(t1);
+        phase_end(t2);
         phase_fence();
         R e2c[4], t3[4];
         {
@@ -773,21 +812,6 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         }
         phase_end(e2c);
         phase_end(t3);
-        phase_fence();
-        R t1[4], t2[4];
-        {
-            R Am[4], Ap[4], Bz[6];
-            load4(&sA[rw * LW + kc], Am);
-            load4(&sA[(rw + 2) * LW + kc], Ap);
-            win6(&sB[(rw + 1) * LW], kc, Bz);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                t1[e] = (Ap[e] - Am[e]) * a.inv_dy;
-                t2[e] = -(Bz[e + 2] - Bz[e]) * a.inv_dx;
-            }
-        }
-        phase_end(t1);
-        phase_end(t2);
         phase_fence();
 
         // the reaction matters only where phi (1 - phi) != 0 (or T is not
@@ -875,11 +899,14 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     }
 }
 
-// FP32: 3 blocks of 8 waves per CU fit the LDS (3 x 46.9 KB); the register
-// allocator is asked for the matching 6 waves per SIMD.  FP64 needs twice the
-// LDS and registers and runs one block per CU.
+// FP32: 4 blocks of 8 waves per CU fit the LDS (4 x 37.5 KB); the register
+// allocator is asked for the matching 8 waves per SIMD (at most 64 VGPRs).
+// With fewer waves asked for, the allocator is handed the registers the LDS
+// limit leaves unused anyway (3 blocks: 6 waves, up to 80 VGPRs), which is
+// why the kernel descriptor then shows more VGPRs than the code needs.
+// FP64 needs twice the LDS and registers and runs two blocks per CU.
 #ifndef KG_WAVES_PER_EU
-#define KG_WAVES_PER_EU 6
+#define KG_WAVES_PER_EU 8
 #endif
 template <typename R> struct Occupancy;
 template <> struct Occupancy<float> { static constexpr int waves = KG_WAVES_PER_EU; };
@@ -1018,6 +1045,20 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
     if (!chunk_tile((int)blockIdx.x - q.blocks, tiles_x, tiles_y, tx, ty, c))
         return;
     step_tile<R, BC>(a, tx * TX, ty * TY, sm);
+}
+
+// The step without tile status: every tile loads and stores all of its
+// fields (the same results as k_step_fused, whose status only proves some of
+// that work unneeded).
+template <typename R, int BC>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_dense(StepArgs<R> a)
+{
+    __shared__ __align__(16) TileSmem<R> sm;
+    const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
+    int tx, ty, c;
+    if (!chunk_tile((int)blockIdx.x, tiles_x, tiles_y, tx, ty, c))
+        return;
+    step_tile<R, BC, false>(a, tx * TX, ty * TY, sm);
 }
 
 // The same step over the chunks of `live` only (a separate kernel, so that
@@ -1299,45 +1340,64 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *part
 }
 
 // ---- display ramp (src/Kobayashi.cpp:309-345) -----------------------------
-template <typename R>
-__global__ void k_colormap(const R *phi_f, int nx, int ny, int order, float4 *rgba)
+// Colour of phi: three linear segments with breaks at 0.9, 0.99 and 1.0
+// between the reference's four colours (:318-325), in its float arithmetic.
+__device__ __forceinline__ float3 ramp(float phi)
+{
+    const float c0[3] = {0.0f, 0.0f, 0.0f};
+    const float c1[3] = {0.2505490f, 0.5000000f, 0.9882353f};
+    const float c2[3] = {0.3607843f, 1.0000000f, 0.9882353f};
+    const float c3[3] = {0.9005490f, 1.0000000f, 0.9882353f};
+    const float b1 = 0.9f, b2 = 0.99f, b3 = 1.0f;
+    const float *lo, *hi;
+    float ratio;
+    if (phi <= b1) {
+        ratio = phi * (1.0f / b1);
+        lo = c0;
+        hi = c1;
+    } else if (phi > b1 && phi <= b2) {
+        ratio = (phi - b1) * (1.0f / (b2 - b1));
+        lo = c1;
+        hi = c2;
+    } else {
+        ratio = (phi - b2) * (1.0f / (b3 - b2));
+        lo = c2;
+        hi = c3;
+    }
+    return make_float3(lo[0] * (1.0f - ratio) + hi[0] * ratio,
+                       lo[1] * (1.0f - ratio) + hi[1] * ratio,
+                       lo[2] * (1.0f - ratio) + hi[2] * ratio);
+}
+
+// float -> 8-bit unsigned normalised: clamp to [0, 1], scale by 255, round
+// to the nearest integer (ties to even)
+__device__ __forceinline__ uint32_t unorm8(float c)
+{
+    c = fminf(fmaxf(c, 0.0f), 1.0f);
+    return (uint32_t)rintf(c * 255.0f);
+}
+
+// order 0: entry c is cell c (x fastest); order 1: the reference's constant
+// buffer order, entry c showing _phi[_INDEX(c / n, c % n)] (square grids).
+// RGBA8 false: 4 floats per entry (alpha 1); true: 4 bytes R, G, B, A = 255.
+template <typename R, bool RGBA8>
+__global__ void k_colormap(const R *phi_f, int nx, int ny, int order, void *out)
 {
     const size_t n = (size_t)nx * ny;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
          c += (size_t)gridDim.x * blockDim.x) {
         size_t src = c;
         if (order == 1) {
-            // constant buffer entry i shows _phi[_INDEX(i / n, i % n)]
             const size_t side = (size_t)nx;
             src = (c / side) + (size_t)nx * (c % side);
         }
-        const float phi = (float)phi_f[src];
-        const float c0[3] = {0.0f, 0.0f, 0.0f};
-        const float c1[3] = {0.2505490f, 0.5000000f, 0.9882353f};
-        const float c2[3] = {0.3607843f, 1.0000000f, 0.9882353f};
-        const float c3[3] = {0.9005490f, 1.0000000f, 0.9882353f};
-        const float b1 = 0.9f, b2 = 0.99f, b3 = 1.0f;
-        const float *lo, *hi;
-        float ratio;
-        if (phi <= b1) {
-            ratio = phi * (1.0f / b1);
-            lo = c0;
-            hi = c1;
-        } else if (phi > b1 && phi <= b2) {
-            ratio = (phi - b1) * (1.0f / (b2 - b1));
-            lo = c1;
-            hi = c2;
+        const float3 col = ramp((float)phi_f[src]);
+        if (RGBA8) {
+            const uint32_t w = unorm8(col.x) | unorm8(col.y) << 8 | unorm8(col.z) << 16 | 255u << 24;
+            static_cast<uint32_t *>(out)[c] = w;
         } else {
-            ratio = (phi - b2) * (1.0f / (b3 - b2));
-            lo = c2;
-            hi = c3;
+            static_cast<float4 *>(out)[c] = make_float4(col.x, col.y, col.z, 1.0f);
         }
-        float4 o;
-        o.x = lo[0] * (1.0f - ratio) + hi[0] * ratio;
-        o.y = lo[1] * (1.0f - ratio) + hi[1] * ratio;
-        o.z = lo[2] * (1.0f - ratio) + hi[2] * ratio;
-        o.w = 1.0f;
-        rgba[c] = o;
     }
 }
 
@@ -1379,6 +1439,14 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList
         else
             hipLaunchKernelGGL((k_step_fused_listed<R, 0>), dim3(nblocks), dim3(NT), 0, s, b, *live,
                                q);
+        return hipGetLastError();
+    }
+    if (!a.stat_in) {
+        const int nblocks = chunk_blocks(tiles_x, tiles_y);
+        if (a.boundary == 1)
+            hipLaunchKernelGGL((k_step_dense<R, 1>), dim3(nblocks), dim3(NT), 0, s, b);
+        else
+            hipLaunchKernelGGL((k_step_dense<R, 0>), dim3(nblocks), dim3(NT), 0, s, b);
         return hipGetLastError();
     }
     const int nblocks = chunk_blocks(tiles_x, tiles_y) + q.blocks;
@@ -1518,11 +1586,16 @@ hipError_t launch_reduce(const R *phi, const R *T, size_t n, double K, double *p
 }
 
 template <typename R>
-hipError_t launch_colormap(const R *phi, int nx, int ny, int order, float *rgba, hipStream_t s)
+hipError_t launch_colormap(const R *phi, int nx, int ny, int order, void *rgba, bool rgba8,
+                           hipStream_t s)
 {
     const size_t n = (size_t)nx * ny;
-    hipLaunchKernelGGL((k_colormap<R>), dim3(grid_1d(n, 256)), dim3(256), 0, s, phi, nx, ny, order,
-                       reinterpret_cast<float4 *>(rgba));
+    if (rgba8)
+        hipLaunchKernelGGL((k_colormap<R, true>), dim3(grid_1d(n, 256)), dim3(256), 0, s, phi, nx,
+                           ny, order, rgba);
+    else
+        hipLaunchKernelGGL((k_colormap<R, false>), dim3(grid_1d(n, 256)), dim3(256), 0, s, phi, nx,
+                           ny, order, rgba);
     return hipGetLastError();
 }
 
@@ -1597,7 +1670,7 @@ hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double 
                                        double, hipStream_t);                                     \
     template hipError_t launch_reduce<R>(const R *, const R *, size_t, double, double *,         \
                                          ReduceOut *, hipStream_t);                              \
-    template hipError_t launch_colormap<R>(const R *, int, int, int, float *, hipStream_t);       \
+    template hipError_t launch_colormap<R>(const R *, int, int, int, void *, bool, hipStream_t);  \
     template hipError_t launch_front_profile<R>(const R *, int, int, double, double, int, double,  \
                                                 double *, hipStream_t);
 KG_INSTANTIATE(float)

@@ -200,6 +200,13 @@ class Solver:
         check(lib().kg_colormap(self._h, out.ctypes.data_as(ctypes.c_void_p), out.nbytes, order))
         return out
 
+    def colormap_rgba8(self, order: int = 0) -> np.ndarray:
+        """The colours as RGBA8 (kg_colormap_rgba8), shape (cells, 4) uint8."""
+        out = np.empty((self.nx * self.ny, 4), np.uint8)
+        check(lib().kg_colormap_rgba8(self._h, out.ctypes.data_as(ctypes.c_void_p), out.nbytes,
+                                      order))
+        return out
+
     # -- adjoint ----------------------------------------------------------
     def adjoint(self, nsteps: int, target, checkpoint_every: int = 50, want_T0: bool = False,
                 want_phi0: bool = False):

@@ -165,6 +165,12 @@ int kg_reduce(kg_handle h, kg_stats *out);
  * fastest; order 1: the reference's constant-buffer order, entry i showing
  * _phi[_INDEX(i / n, i % n)] (square grids only). */
 int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order);
+/* The same colours as 8-bit unsigned normalised RGBA, 4 bytes per cell (R, G,
+ * B, A = 255; each channel 255 c rounded to the nearest integer, ties to
+ * even, c clamped to [0, 1]): the float-to-UNORM8 conversion a viewer applies
+ * to the constant buffer's float colour (src/Kobayashi.cpp:344), done on the
+ * device so a frame crosses PCIe at a quarter of the float size. */
+int kg_colormap_rgba8(kg_handle h, uint8_t *rgba, size_t bytes, int order);
 /* Morphology of the phi = level front (no counterpart in the reference, which
  * only draws the field; the long-horizon comparisons use it): radii[k] = the
  * distance in cells from (cx, cy) to the first point where phi < level along

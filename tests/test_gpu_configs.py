@@ -1,0 +1,109 @@
+"""The benchmark configurations at their own sizes, against the CPU oracle
+(needs an MI355X).
+
+  c0  the reference's configuration: 300^2 FP64, 2000 steps, disk r = 5,
+      zero-flux, noise 0.01 -- every cell within 1e-10 of the FP64 oracle.
+  c2  the 512-member ensemble (sides 128..1024, per-member delta, K, j,
+      theta0, eps_bar, zero-flux): eight members spanning the sides and all
+      three j, after 100 batched steps, against the FP32 and the FP64 oracle
+      (relative L2 <= 1e-5 on phi and on T), and the same
+      eight members with periodic boundaries; all 512 members bit for bit
+      equal to plain handles on the dense kernel after 1000 steps.
+The oracle runs on the host's cores (its OpenMP build, bitwise the serial
+oracle: tests/test_oracle_parallel.py).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def kb():
+    import crystalgrowth_amd as kb
+    if kb.device_count() < 1:
+        pytest.fail("no GPU visible")
+    return kb
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b))
+
+
+def oracle_params(O, p):
+    return O.default_params(**p.as_dict())
+
+
+def test_c0_reference_configuration_per_cell(kb, O):
+    from crystalgrowth_amd import configs
+    c = configs.C0
+    n, steps = c["n"], c["steps"]
+    p = configs.c0_params()
+    with kb.Solver(n, n, "f64", p) as s:
+        s.reset_disk(n // 2, n // 2, c["disk_r"])
+        s.step(steps)
+        phi, T, th = s.fields()
+    ref = O.run_parallel(O.init_disk(n, n, n // 2, n // 2, c["disk_r"], np.float64),
+                         oracle_params(O, p), steps)
+    assert np.abs(phi - ref.phi).max() <= 1e-10
+    assert np.abs(T - ref.T).max() <= 1e-10
+    assert (phi > 0.5).mean() > 0.05          # the crystal has grown
+
+
+def c2_sample(sizes, params):
+    """Eight members: sides spread over 128..1024, every j of {4, 5, 6}."""
+    order = sorted(range(len(sizes)), key=lambda m: (sizes[m][0], m))
+    picks = [order[int(k)] for k in np.linspace(0, len(order) - 1, 8)]
+    for j in (4.0, 5.0, 6.0):
+        if not any(params[m].aniso == j for m in picks):
+            picks[-2] = next(m for m in order[::-1] if params[m].aniso == j and m not in picks)
+    return picks
+
+
+@pytest.mark.parametrize("boundary", [1, 0])
+def test_c2_members_match_oracle(kb, O, boundary):
+    from crystalgrowth_amd import configs
+    sizes, params = configs.c2_members()
+    picks = c2_sample(sizes, params)
+    assert {params[m].aniso for m in picks} == {4.0, 5.0, 6.0}
+    assert min(sizes[m][0] for m in picks) == 128 and max(sizes[m][0] for m in picks) == 1024
+    steps = 100
+    if boundary == 1:
+        # the configuration itself: all 512 members in one batch
+        bsizes, bparams, idx = sizes, params, picks
+    else:
+        bsizes = [sizes[m] for m in picks]
+        bparams = [params[m].replace(boundary=0) for m in picks]
+        idx = list(range(len(picks)))
+    with kb.Batch(bsizes, bparams, "f32") as b:
+        b.reset_disk(-1, -1, -1, 5.0)
+        b.step(steps)
+        got = {m: (b.phase(m), b.temperature(m)) for m in idx}
+    for k, m in zip(idx, picks):
+        nx, ny = bsizes[k]
+        op = oracle_params(O, bparams[k])
+        s0 = O.init_disk(nx, ny, nx // 2, ny // 2, 5.0, np.float64)
+        r64 = O.run_parallel(s0, op, steps)
+        r32 = O.run_parallel(s0.astype(np.float32), op, steps)
+        phi, T = got[k]
+        for name, g, a, b64 in (("phi", phi, r32.phi, r64.phi), ("T", T, r32.T, r64.T)):
+            assert rel(g, b64) <= 1e-5, (m, nx, name, rel(g, b64))
+            assert rel(g, a.astype(np.float64)) <= 1e-5, (m, nx, name)
+
+
+def test_c2_all_members_match_plain_dense_handles_bitwise(kb):
+    from crystalgrowth_amd import configs
+    sizes, params = configs.c2_members()
+    steps = 1000
+    with kb.Batch(sizes, params, "f32") as b:
+        b.reset_disk(-1, -1, -1, 5.0)
+        b.step(steps)
+        stats = b.skip_stats()
+        for m, ((nx, ny), p) in enumerate(zip(sizes, params)):
+            with kb.Solver(nx, ny, "f32", p) as s:
+                s.set_kernel("fused_dense")
+                s.reset_disk(nx // 2, ny // 2, 5.0)
+                s.step(steps)
+                for name, x, y in zip(("phi", "T", "theta"), b.fields(m), s.fields()):
+                    assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), (m, nx, name)
+    assert stats["listed_launches"] > 0, stats   # chunk lists took part

@@ -1,6 +1,8 @@
 """The headless C++ driver (csrc/kobayashi_headless.cpp: the reference's
-WinMain loop without a window, on the C++ Kobayashi class over the C ABI)
-runs on the GPU and ends where the Python class over the same ABI ends."""
+WinMain loop without a window, on the C++ Kobayashi class that implements
+ISimulation over the C ABI) runs on the GPU and ends where the Python class
+over the same ABI ends; parameter flags and config files act as the GUI's
+sliders (reset, warning outside the slider ranges)."""
 import json
 import os
 import subprocess
@@ -14,30 +16,73 @@ BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                    "crystalgrowth_amd", "bin", "kobayashi_headless")
 
 
-def test_headless_driver_matches_python_class(tmp_path):
+@pytest.fixture(scope="module")
+def kb():
     import crystalgrowth_amd as kb
     if kb.device_count() < 1:
         pytest.fail("no GPU visible")
     if not os.path.exists(BIN):
         pytest.fail("headless driver not built (crystalgrowth_amd/build.py)")
-    ppm = tmp_path / "frame.ppm"
-    r = subprocess.run([BIN, "--size", "64", "--frames", "20", "--ppm", str(ppm)],
-                       capture_output=True, text=True, timeout=120)
+    return kb
+
+
+def run(*args):
+    r = subprocess.run([BIN, *map(str, args)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    out = json.loads(r.stdout.strip().splitlines()[-1])
-    assert out["size"] == 64 and out["frames"] == 20 and out["steps"] == 200
-    sim = kb.Kobayashi(64, 64, 1e-4)
+    return json.loads(r.stdout.strip().splitlines()[-1]), r.stderr
+
+
+def python_run(kb, n, frames, **params):
+    sim = kb.Kobayashi(n, n, 1e-4)
     try:
-        for _ in range(20):
+        if params:
+            sim.params = sim.params.replace(**params)
+            sim.iResetSimulationState()
+        for _ in range(frames):
             sim.iUpdate()
-        st = sim.stats()
+        return sim.stats(), sim.iUpdateConstantBuffer()
     finally:
         sim.close()
+
+
+def test_headless_driver_matches_python_class(kb, tmp_path):
+    ppm = tmp_path / "frame.ppm"
+    out, _ = run("--size", 64, "--frames", 20, "--ppm", ppm)
+    assert out["nx"] == 64 and out["frames"] == 20 and out["steps"] == 200
+    assert out["constant_buffers"] == 64 * 64 and out["out_of_reference_range"] == []
+    st, colors = python_run(kb, 64, 20)
     assert out["sum_phi"] == st["sum_phi"]
     assert out["sum_T_minus_K_phi"] == st["sum_T_minus_K_phi"]
     assert out["nonfinite"] == 0
+    # every entry coloured through iUpdateConstantBuffer(cb, i)
+    assert out["color_sum"] == pytest.approx(float(colors[:, :3].astype(np.float64).sum()),
+                                             rel=1e-12)
     data = ppm.read_bytes()
     header = b"P6\n64 64\n255\n"
     assert data.startswith(header) and len(data) == len(header) + 64 * 64 * 3
     px = np.frombuffer(data[len(header):], np.uint8)
     assert px.max() > px.min()   # the nucleus shows
+
+
+def test_parameter_flags_and_config_file(kb, tmp_path):
+    """Flags and a config file set the slider parameters and the model
+    extensions; the run equals the Python class with the same parameters."""
+    cfg = tmp_path / "run.cfg"
+    cfg.write_text("# slider values\ndelta = 0.04\naniso = 4   # j\nK = 1.8\n"
+                   "theta0 = 0.2\nnoise = 0.01\nseed = 7\nboundary = zero-flux\nframes = 15\n")
+    out, err = run("--size", 80, "--config", cfg, "--K", 1.5)   # the flag overrides the file
+    assert (out["delta"], out["aniso"], out["K"], out["boundary"]) == (0.04, 4.0, 1.5, 1)
+    assert out["frames"] == 15 and out["out_of_reference_range"] == [] and err == ""
+    st, _ = python_run(kb, 80, 15, delta=0.04, aniso=4.0, K=1.5, theta0=0.2, noise_a=0.01,
+                       seed=7, boundary=1)
+    assert out["sum_phi"] == st["sum_phi"]
+    assert out["solid_fraction"] == st["solid_fraction"]
+
+
+def test_out_of_range_parameters_warn(kb):
+    """A value beyond the GUI's slider (delta 0.01..0.09, K 1.0..1.9:
+    src/Kobayashi.cpp:17-58) is used, with a warning naming it."""
+    out, err = run("--size", 48, "--frames", 2, "--delta", 0.12, "--K", 2.5)
+    assert out["out_of_reference_range"] == ["K", "delta"]
+    assert "delta is outside" in err and "K is outside" in err
+    assert out["delta"] == 0.12 and out["K"] == 2.5 and out["nonfinite"] == 0

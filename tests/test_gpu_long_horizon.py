@@ -49,3 +49,36 @@ def test_fp64_disk_run_per_cell_2000_steps(kb, O):
         phi, T, _ = g.fields()
     assert np.abs(phi - ref.phi).max() <= 1e-9
     assert np.abs(T - ref.T).max() <= 1e-9
+
+
+def test_c1_physics_morphology_tip_velocity(kb, O):
+    """The 4096^2 configuration's physics (zero-flux, j = 6, delta = 0.04,
+    K = 2.0, theta0 = 0.2, noise 0.01) on 768^2 from one r = 5 disk at the
+    centre: FP32 GPU against the FP32 oracle after 3000 and 4000 steps.
+    Tolerances: solid fraction within 2 %, tip radius within 3 %, tip velocity
+    (3000 -> 4000) within 5 %, the same arm count (6)."""
+    n, s1, s2 = 768, 3000, 4000
+    p = O.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1,
+                         seed=0)
+    a = O.run_parallel(O.init_disk(n, n, n // 2, n // 2, 5.0, np.float32), p, s1)
+    b = O.run_parallel(a, p, s2 - s1, step0=s1)
+    with kb.Solver(n, n, "f32", kb.Params(**p.as_dict())) as g:
+        g.reset_disk(n // 2, n // 2, 5.0)
+        g.step(s1)
+        ga = g.phase()
+        g.step(s2 - s1)
+        gb = g.phase()
+        dev = M.device_summary(g, n // 2, n // 2)
+    c = n // 2
+    for got, ref in ((ga, a.phi), (gb, b.phi)):
+        x, y = M.summary(got, c, c), M.summary(ref, c, c)
+        assert abs(x["solid_fraction"] - y["solid_fraction"]) <= 0.02 * y["solid_fraction"]
+        assert abs(x["tip_radius"] - y["tip_radius"]) <= 0.03 * y["tip_radius"]
+        assert x["arm_count"] == y["arm_count"] == 6
+    span = (s2 - s1) * p.dt
+    vg = M.tip_velocity(ga, gb, c, c, span, p.dx)
+    vo = M.tip_velocity(a.phi, b.phi, c, c, span, p.dx)
+    assert vo > 0 and abs(vg - vo) <= 0.05 * vo, (vg, vo)
+    # the device's morphology of the final state agrees with the host's
+    assert dev["arm_count"] == 6
+    assert dev["tip_radius"] == pytest.approx(M.tip_radius(gb, c, c), abs=0.3)

@@ -298,3 +298,22 @@ def test_set_get_and_step_counter(kb):
             s.set_fields(a[:5], b)
         with pytest.raises(kb.KGError):
             s.set_params(dt=1.0)
+
+
+def test_colormap_rgba8_is_unorm8_of_float_ramp(kb):
+    """kg_colormap_rgba8: each channel of kg_colormap's float colour, clamped
+    to [0, 1], times 255 rounded to nearest (ties to even); alpha 255; both
+    entry orders."""
+    n = 48
+    rng = np.random.default_rng(3)
+    with kb.Solver(n, n, "f32") as s:
+        phi = rng.random((n, n)).astype(np.float32)
+        phi[0, :8] = [0.0, 0.9, 0.95, 0.99, 0.995, 1.0, -0.2, 1.3]  # breaks and clamps
+        s.set_fields(phi, np.zeros((n, n), np.float32))
+        for order in (0, 1):
+            f = s.colormap(order)
+            b = s.colormap_rgba8(order)
+            want = np.rint(np.clip(f[:, :3], 0, 1) * np.float32(255))
+            assert b.dtype == np.uint8 and b.shape == (n * n, 4)
+            assert np.array_equal(b[:, :3], want.astype(np.uint8))
+            assert (b[:, 3] == 255).all()
