@@ -92,11 +92,26 @@ __device__ __forceinline__ void store4(float *p, const float v[4])
 {
     *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
 }
+// Stores of step outputs, which this launch never reads back.
+#ifndef KG_NT_STORES
+#define KG_NT_STORES 0
+#endif
+typedef float kg_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store4_out(float *p, const float v[4])
+{
+    if (KG_NT_STORES) {
+        const kg_f4 q = {v[0], v[1], v[2], v[3]};
+        __builtin_nontemporal_store(q, reinterpret_cast<kg_f4 *>(p));
+    } else {
+        store4(p, v);
+    }
+}
 __device__ __forceinline__ void store4(double *p, const double v[4])
 {
     reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
     reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
 }
+__device__ __forceinline__ void store4_out(double *p, const double v[4]) { store4(p, v); }
 
 // all four exactly +0 / all four exactly 1 (bit patterns, so -0 is not flat)
 __device__ __forceinline__ void flat4(const float v[4], int &all0, int &all1)
@@ -250,29 +265,42 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
 // reuse between neighbouring chunks.  With launches of the live chunks only
 // (below) the smaller chunk skips more: 40.7 us at 2 x 8, 40.1 us at 2 x 4.
 constexpr int CHUNK_W = 2, CHUNK_H = 4, CHUNK_SKEW = 3;
+// The dense launch (no status, so no chunk lists) deals larger chunks: a
+// tile row reads a 16-byte segment on each side of its 64 columns, i.e. a
+// 128-byte line of each neighbour, so a chunk's left and right halo lines
+// add 2 lines per chunk row to its own 2 per tile; wider chunks share more
+// of those lines in one XCD's L2.
+#ifndef KG_DENSE_CHUNK_W
+#define KG_DENSE_CHUNK_W 2
+#endif
+#ifndef KG_DENSE_CHUNK_H
+#define KG_DENSE_CHUNK_H 4
+#endif
+constexpr int DENSE_CHUNK_W = KG_DENSE_CHUNK_W, DENSE_CHUNK_H = KG_DENSE_CHUNK_H;
 
-// Tile (tx, ty) of tile block b and its chunk's row-major index c; false
-// for blocks past the grid.
+// Tile (tx, ty) of tile block b and its chunk's row-major index c (chunks
+// of CW x CH tiles); false for blocks past the grid.
+template <int CW = CHUNK_W, int CH = CHUNK_H>
 __device__ __forceinline__ bool chunk_tile(int b, int tiles_x, int tiles_y, int &tx, int &ty,
                                            int &c)
 {
     const int i = b / NXCD;
-    const int chunk = (i / (CHUNK_W * CHUNK_H)) * NXCD + b % NXCD;
-    const int k = i % (CHUNK_W * CHUNK_H);
-    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
+    const int chunk = (i / (CW * CH)) * NXCD + b % NXCD;
+    const int k = i % (CW * CH);
+    const int ncx = (tiles_x + CW - 1) / CW;
     const int cy = chunk / ncx;
     const int cx = (chunk - cy * ncx + CHUNK_SKEW * cy) % ncx;  // a rotation of the row
-    tx = cx * CHUNK_W + k % CHUNK_W;
-    ty = cy * CHUNK_H + k / CHUNK_W;
+    tx = cx * CW + k % CW;
+    ty = cy * CH + k / CW;
     c = cy * ncx + cx;
     return tx < tiles_x && ty < tiles_y;
 }
 
 // blocks of the single-grid launch: whole chunk rounds over the 8 XCDs
-int chunk_blocks(int tiles_x, int tiles_y)
+int chunk_blocks(int tiles_x, int tiles_y, int cw = CHUNK_W, int ch = CHUNK_H)
 {
-    const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W, ncy = (tiles_y + CHUNK_H - 1) / CHUNK_H;
-    return (ncx * ncy + NXCD - 1) / NXCD * NXCD * CHUNK_W * CHUNK_H;
+    const int ncx = (tiles_x + cw - 1) / cw, ncy = (tiles_y + ch - 1) / ch;
+    return (ncx * ncy + NXCD - 1) / NXCD * NXCD * cw * ch;
 }
 
 // Tile status: one word per tile and buffer, written by the block that
@@ -854,8 +882,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     const size_t o = (size_t)j * nx + i0;
     if (i0 + 3 < nx && (nx & 3) == 0) {
         if (w_phi)
-            store4(a.phi_out + o, phin);
-        store4(a.T_out + o, Tn);
+            store4_out(a.phi_out + o, phin);
+        store4_out(a.T_out + o, Tn);
     } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -1056,7 +1084,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
     __shared__ __align__(16) TileSmem<R> sm;
     const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
     int tx, ty, c;
-    if (!chunk_tile((int)blockIdx.x, tiles_x, tiles_y, tx, ty, c))
+    if (!chunk_tile<DENSE_CHUNK_W, DENSE_CHUNK_H>((int)blockIdx.x, tiles_x, tiles_y, tx, ty, c))
         return;
     step_tile<R, BC, false>(a, tx * TX, ty * TY, sm);
 }
@@ -1442,7 +1470,7 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList
         return hipGetLastError();
     }
     if (!a.stat_in) {
-        const int nblocks = chunk_blocks(tiles_x, tiles_y);
+        const int nblocks = chunk_blocks(tiles_x, tiles_y, DENSE_CHUNK_W, DENSE_CHUNK_H);
         if (a.boundary == 1)
             hipLaunchKernelGGL((k_step_dense<R, 1>), dim3(nblocks), dim3(NT), 0, s, b);
         else

@@ -27,6 +27,15 @@ LR = {"delta": 1e-3, "K": 0.03, "theta0": 0.03}
 BOUNDS = {"delta": (0.0, 0.2), "K": (0.2, 4.0), "theta0": (-np.pi, np.pi)}
 
 
+def lr_at(lr: dict, it: int, iterations: int, schedule: str) -> dict:
+    """Learning rates of iteration `it` (1-based): constant, or a cosine decay
+    to a tenth of the initial rate over the run."""
+    if schedule == "constant" or iterations <= 1:
+        return dict(lr)
+    f = 0.1 + 0.9 * 0.5 * (1.0 + np.cos(np.pi * (it - 1) / (iterations - 1)))
+    return {k: v * f for k, v in lr.items()}
+
+
 def make_targets(sizes, hidden, steps, precision, r=5.0):
     with Batch(sizes, hidden, precision) as b:
         b.reset_disk(-1, -1, -1, r)
@@ -34,7 +43,11 @@ def make_targets(sizes, hidden, steps, precision, r=5.0):
         return [b.phase(m) for m in range(len(sizes))]
 
 
-def fit(sizes, hidden, guess, steps, iterations, every=50, precision="f32", r=5.0, log=None):
+def fit(sizes, hidden, guess, steps, iterations, every=50, precision="f32", r=5.0, log=None,
+        lr=None, schedule="constant"):
+    """Adam on (delta, K, theta0) of every member; `lr` maps each name to its
+    initial learning rate (default LR), `schedule` is "constant" or "cosine"."""
+    lr = dict(LR if lr is None else lr)
     targets = make_targets(sizes, hidden, steps, precision, r)
     cur = [g.replace() for g in guess]
     M = len(sizes)
@@ -52,6 +65,7 @@ def fit(sizes, hidden, guess, steps, iterations, every=50, precision="f32", r=5.
             grads, _ = b.adjoint(steps, targets, every)
             t_adj += time.perf_counter() - t0
             losses.append(float(sum(g["loss"] for g in grads)))
+            rate = lr_at(lr, it, iterations, schedule)
             for m in range(M):
                 g = np.array([grads[m]["d_" + n] for n in NAMES])
                 m1[m] = b1 * m1[m] + (1 - b1) * g
@@ -60,7 +74,7 @@ def fit(sizes, hidden, guess, steps, iterations, every=50, precision="f32", r=5.
                 vh = m2[m] / (1 - b2 ** it)
                 upd = {}
                 for k, n in enumerate(NAMES):
-                    v = getattr(cur[m], n) - LR[n] * mh[k] / (np.sqrt(vh[k]) + eps)
+                    v = getattr(cur[m], n) - rate[n] * mh[k] / (np.sqrt(vh[k]) + eps)
                     lo, hi = BOUNDS[n]
                     upd[n] = float(min(max(v, lo), hi))
                 cur[m] = cur[m].replace(**upd)
