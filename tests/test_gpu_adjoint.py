@@ -188,3 +188,50 @@ def test_batched_adjoint_matches_single_handles(kb):
             g, dT, _ = s.adjoint(30, c[3], checkpoint_every=8, want_T0=True)
         assert g == gb[m], m
         assert np.array_equal(dT, dTb[m])
+
+
+def test_c3_fp32_adjoint_at_its_size_matches_fp64_central_differences(kb):
+    """The adjoint configuration itself: 512^2 FP32, 1000 steps, checkpoint
+    every 50, target from delta 0.04, K 1.8 (crystalgrowth_amd/configs.py c3).
+    dL/d(delta, K, eps_bar, tau) and dL/dT0 at four cells (the three with
+    the largest FP64 sensitivity and one off the front) of the FP32 adjoint
+    against central differences (h = 1e-4, relative for the parameters,
+    absolute for T0) of the FP64 forward: <= 1e-3 relative.  Measured on
+    MI355X: <= 4e-5 for the parameters, <= 1.7e-5 for the T0 cells."""
+    from crystalgrowth_amd import configs
+    c = configs.C3
+    n, steps, every = c["n"], c["steps"], c["every"]
+    with kb.Solver(n, n, "f32", configs.c3_params("target")) as s:
+        s.reset_disk(n // 2, n // 2, 5.0)
+        s.step(steps)
+        target = s.phase().astype(np.float64)
+    base = configs.c3_params("guess")
+    with kb.Solver(n, n, "f32", base) as s:
+        s.reset_disk(n // 2, n // 2, 5.0)
+        g32, dT32, _ = s.adjoint(steps, target.astype(np.float32), every, want_T0=True)
+    with kb.Solver(n, n, "f64", base) as s:
+        s.reset_disk(n // 2, n // 2, 5.0)
+        phi0, T0, _ = s.fields()
+        _, dT64, _ = s.adjoint(steps, target, every, want_T0=True)
+
+    def loss(p, T0_):
+        with kb.Solver(n, n, "f64", p) as s:
+            s.set_fields(phi0, T0_)
+            s.step(steps)
+            d = s.phase() - target
+            return float((d * d).sum())
+
+    for name in ("delta", "K", "eps_bar", "tau"):
+        v = getattr(base, name)
+        h = 1e-4 * abs(v)
+        ref = (loss(base.replace(**{name: v + h}), T0) - loss(base.replace(**{name: v - h}), T0)) / (2 * h)
+        assert abs(g32["d_" + name] - ref) <= 1e-3 * abs(ref), (name, g32["d_" + name], ref)
+    cells = list(np.argsort(np.abs(dT64).ravel())[::-1][:3]) + [n * (n // 2 + 30) + n // 2]
+    h = 1e-4
+    for k in cells:
+        tp, tm = T0.copy(), T0.copy()
+        tp.ravel()[k] += h
+        tm.ravel()[k] -= h
+        ref = (loss(base, tp) - loss(base, tm)) / (2 * h)
+        got = float(dT32.ravel()[k])
+        assert abs(got - ref) <= 1e-3 * abs(ref), (int(k), got, ref)
