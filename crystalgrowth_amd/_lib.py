@@ -122,6 +122,7 @@ SIGNATURES = {
     "kg_reduce": [H, ctypes.POINTER(Stats)],
     "kg_colormap": [H, VP, SZ, I],
     "kg_colormap_rgba8": [H, VP, SZ, I],
+    "kg_colormap_rgba8_noalloc": [H, VP, SZ, I],
     "kg_front_profile": [H, ctypes.c_double, ctypes.c_double, I, ctypes.c_double, VP],
     "kg_adjoint": [H, I, I, VP, SZ, ctypes.POINTER(Grads), VP, VP],
     "kg_batch_create": [I, PI, PI, I, P, ctypes.POINTER(H)],

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -773,6 +774,82 @@ int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order)
 int kg_colormap_rgba8(kg_handle h, uint8_t *rgba, size_t bytes, int order)
 {
     return colormap_impl(h, rgba, bytes, order, true);
+}
+
+}  // extern "C"
+
+// ---- kg_colormap_rgba8_noalloc --------------------------------------------------
+namespace {
+
+// Staging of kg_colormap_rgba8_noalloc: the RGBA8 colours of up to
+// KG_COLORMAP_NOALLOC_MAX_CELLS cells (16 MiB).  Every GPU the library runs
+// on holds its own copy until the process ends.  In host code `scratch` names
+// a host-side stand-in, not a device address: the address of each GPU's copy
+// is taken once per device with hipGetSymbolAddress and kept in
+// g_scratch_ptr, and only that address is ever handed to a kernel or a copy.
+static __device__ __align__(256) uint8_t scratch[(size_t)KG_COLORMAP_NOALLOC_MAX_CELLS * 4];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (device `dev` is current).
+int scratch_address(int dev, void **out)
+{
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(KG_ERR_DEVICE, "device index beyond the library's scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+            return hip_fail(e, "hipGetSymbolAddress of the colour-ramp scratch");
+        if (!p)
+            return fail(KG_ERR_DEVICE, "hipGetSymbolAddress gave no address for the scratch");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return KG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kg_colormap_rgba8_noalloc(kg_handle h, uint8_t *rgba, size_t bytes, int order)
+{
+    if (int rc = check_handle(h))
+        return rc;
+    if (!rgba)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null rgba");
+    if (order != 0 && order != 1)
+        return fail(KG_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
+    if (order == 1 && h->nx != h->ny)
+        return fail(KG_ERR_INVALID_ARGUMENT, "the reference's buffer order needs a square grid");
+    if (bytes != h->cells() * 4)
+        return fail(KG_ERR_SIZE, "rgba buffer must hold 4 bytes per cell");
+    if (h->cells() > (size_t)KG_COLORMAP_NOALLOC_MAX_CELLS)
+        return fail(KG_ERR_SIZE, "grid of " + std::to_string(h->cells()) +
+                                     " cells exceeds the no-allocation colour ramp's maximum of " +
+                                     std::to_string((long long)KG_COLORMAP_NOALLOC_MAX_CELLS) +
+                                     " cells (use kg_colormap_rgba8)");
+    KG_HIP(hipSetDevice(h->device));
+    void *d = nullptr;
+    if (int rc = scratch_address(h->device, &d))
+        return rc;
+    // every cell's 4 bytes are written by the ramp, so nothing is cleared
+    hipError_t e = h->precision == KG_F64
+                       ? kg::launch_colormap<double>(static_cast<const double *>(h->phi[h->cur]),
+                                                     h->nx, h->ny, order, d, true, h->stream)
+                       : kg::launch_colormap<float>(static_cast<const float *>(h->phi[h->cur]),
+                                                    h->nx, h->ny, order, d, true, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rgba, d, bytes, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess)
+        return hip_fail(e, "colormap (no allocation)");
+    return KG_OK;
 }
 
 int kg_front_profile(kg_handle h, double cx, double cy, int n, double level, double *radii)

@@ -207,6 +207,18 @@ class Solver:
                                       order))
         return out
 
+    def colormap_rgba8_noalloc(self, order: int = 0, out: np.ndarray | None = None) -> np.ndarray:
+        """kg_colormap_rgba8 without device allocations (kg_colormap_rgba8_noalloc):
+        grids up to KG_COLORMAP_NOALLOC_MAX_CELLS cells; `out` may be reused."""
+        if out is None:
+            out = np.empty((self.nx * self.ny, 4), np.uint8)
+        if out.dtype != np.uint8 or out.nbytes != self.nx * self.ny * 4 or \
+                not out.flags["C_CONTIGUOUS"]:
+            raise ValueError("out must be a C-contiguous uint8 array of 4 bytes per cell")
+        check(lib().kg_colormap_rgba8_noalloc(self._h, out.ctypes.data_as(ctypes.c_void_p),
+                                              out.nbytes, order))
+        return out
+
     # -- adjoint ----------------------------------------------------------
     def adjoint(self, nsteps: int, target, checkpoint_every: int = 50, want_T0: bool = False,
                 want_phi0: bool = False):

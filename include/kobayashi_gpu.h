@@ -171,6 +171,18 @@ int kg_colormap(kg_handle h, float *rgba, size_t bytes, int order);
  * to the constant buffer's float colour (src/Kobayashi.cpp:344), done on the
  * device so a frame crosses PCIe at a quarter of the float size. */
 int kg_colormap_rgba8(kg_handle h, uint8_t *rgba, size_t bytes, int order);
+/* kg_colormap_rgba8 without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (render loops
+ * that must not call hipMalloc per frame, code under stream capture,
+ * harnesses that forbid allocation).  The colours are staged in a buffer of
+ * the library's own, KG_COLORMAP_NOALLOC_MAX_CELLS * 4 bytes (16 MiB), that
+ * every GPU it runs on holds until the process ends; larger grids return
+ * KG_ERR_SIZE.  All calls on one GPU share that buffer: calls that overlap on
+ * one GPU (from several threads, or on handles whose streams run
+ * concurrently) race and must be serialised by the caller.  Same results as
+ * kg_colormap_rgba8; synchronous like it. */
+#define KG_COLORMAP_NOALLOC_MAX_CELLS (1 << 22)
+int kg_colormap_rgba8_noalloc(kg_handle h, uint8_t *rgba, size_t bytes, int order);
 /* Morphology of the phi = level front (no counterpart in the reference, which
  * only draws the field; the long-horizon comparisons use it): radii[k] = the
  * distance in cells from (cx, cy) to the first point where phi < level along
