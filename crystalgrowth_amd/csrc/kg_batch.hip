@@ -149,6 +149,8 @@ std::vector<kg::StepArgs<R>> member_args(const kg_batch_s &b)
     for (int m = 0; m < b.count; ++m) {
         kg::StepArgs<R> a = kg::make_args<R>(b.params[m], b.nx[m], b.ny[m]);
         a.wide_angles = kg::wide_angles(b.params[m], b.theta_given[m]);
+        // status launches re-read much of what they wrote from the cache
+        a.stream_out = 0;
         a.phi_in = reinterpret_cast<const R *>(b.field(b.phi[0], m));
         a.T_in = reinterpret_cast<const R *>(b.field(b.T[0], m));
         a.phi_out = reinterpret_cast<R *>(b.field(b.phi[1], m));

@@ -164,6 +164,18 @@ StepArgs<R> make_args(const kg_params &p, int nx, int ny)
     return a;
 }
 
+// StepArgs::stream_out for a launch over `cells` cells of `esize` bytes:
+// whether its per-step working set -- phi and T read and written, theta --
+// exceeds MI355X's 256 MiB last-level (Infinity) cache, so that its outputs
+// cannot be served from cache to the next step anyway.
+#ifndef KG_STREAM_OUT_BYTES
+#define KG_STREAM_OUT_BYTES (256ull << 20)
+#endif
+inline int stream_outputs(size_t cells, size_t esize)
+{
+    return (unsigned long long)cells * esize * 5ull > (unsigned long long)KG_STREAM_OUT_BYTES ? 1 : 0;
+}
+
 // |theta| the step itself writes stays below 2 pi + pi/2 < 8.
 constexpr double STEP_THETA_BOUND = 8.0;
 

@@ -35,6 +35,11 @@ struct StepArgs {
     // 1 when |aniso (theta - theta0)| may exceed the reduced sin/cos range
     // for some stored angle (the host tracks a bound on |theta|)
     int wide_angles;
+    // 1: the step's output stores carry the non-temporal hint (set for
+    // grids whose per-step working set exceeds the last-level cache: the
+    // outputs are not read again within the launch, and without the hint
+    // they evict the input lines neighbouring tiles still read as halos)
+    int stream_out;
     // Tile status words of the fused kernel (one per 64 x 30 tile, bits in
     // kg_step.hip): stat_in describes the input buffers, stat_out the output
     // buffers; the word of the tile at (x0, y0) sits at tile_base +

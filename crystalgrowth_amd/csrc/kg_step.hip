@@ -40,17 +40,20 @@ namespace kg {
 
 namespace {
 
+#ifndef KG_TILE_Y
+#define KG_TILE_Y 30
+#endif
 constexpr int TX = 64;        // output tile width (x, contiguous)
-constexpr int TY = 30;        // output tile height
-constexpr int NT = 512;       // threads per block (8 wave64)
+constexpr int TY = KG_TILE_Y; // output tile height
 constexpr int NG = TX / 4;    // 4-cell groups per row
+constexpr int NT = NG * (TY + 2);  // threads per block: one pass-1 group each (512: 8 wave64)
 constexpr int LW = TX + 8;    // LDS row: columns x0-4 .. x0+67 (16-byte aligned)
 constexpr int PH = TY + 4;    // phi rows y0-2 .. y0+TY+1
 constexpr int QH = TY + 2;    // T, theta and pass-1 rows y0-1 .. y0+TY
 constexpr int NXCD = 8;       // MI355X: workgroups go round-robin over 8 XCDs
 static_assert(NG * QH == NT, "pass 1: one 4-cell group per thread");
 static_assert(NG * TY <= NT, "pass 2: one 4-cell group per thread");
-static_assert(2 * QH <= 64, "pass 1 halo columns: one wave");
+static_assert(2 * QH <= NT && NT % 64 == 0 && NT <= 1024, "pass 1 halo columns: one thread each");
 
 template <typename R> struct Consts;
 template <> struct Consts<float> {
@@ -92,14 +95,14 @@ __device__ __forceinline__ void store4(float *p, const float v[4])
 {
     *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
 }
-// Stores of step outputs, which this launch never reads back.
-#ifndef KG_NT_STORES
-#define KG_NT_STORES 0
-#endif
+// Stores of step outputs, which this launch never reads back: with nt
+// (StepArgs::stream_out, uniform over the launch) they carry the
+// non-temporal hint, so they do not evict input lines that neighbouring
+// tiles still read as halos.
 typedef float kg_f4 __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ void store4_out(float *p, const float v[4])
+__device__ __forceinline__ void store4_out(float *p, const float v[4], int nt)
 {
-    if (KG_NT_STORES) {
+    if (nt) {
         const kg_f4 q = {v[0], v[1], v[2], v[3]};
         __builtin_nontemporal_store(q, reinterpret_cast<kg_f4 *>(p));
     } else {
@@ -111,7 +114,17 @@ __device__ __forceinline__ void store4(double *p, const double v[4])
     reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
     reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
 }
-__device__ __forceinline__ void store4_out(double *p, const double v[4]) { store4(p, v); }
+__device__ __forceinline__ void store4_out(double *p, const double v[4], int nt)
+{
+    if (nt) {
+        __builtin_nontemporal_store(v[0], p);
+        __builtin_nontemporal_store(v[1], p + 1);
+        __builtin_nontemporal_store(v[2], p + 2);
+        __builtin_nontemporal_store(v[3], p + 3);
+    } else {
+        store4(p, v);
+    }
+}
 
 // all four exactly +0 / all four exactly 1 (bit patterns, so -0 is not flat)
 __device__ __forceinline__ void flat4(const float v[4], int &all0, int &all1)
@@ -271,7 +284,7 @@ constexpr int CHUNK_W = 2, CHUNK_H = 4, CHUNK_SKEW = 3;
 // add 2 lines per chunk row to its own 2 per tile; wider chunks share more
 // of those lines in one XCD's L2.
 #ifndef KG_DENSE_CHUNK_W
-#define KG_DENSE_CHUNK_W 2
+#define KG_DENSE_CHUNK_W 8
 #endif
 #ifndef KG_DENSE_CHUNK_H
 #define KG_DENSE_CHUNK_H 4
@@ -572,9 +585,9 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                     const size_t o = (size_t)j * nx + i0;
                     if (i0 + 3 < nx && (nx & 3) == 0) {
                         if (w_phi)
-                            store4(a.phi_out + o, zero4);
+                            store4_out(a.phi_out + o, zero4, a.stream_out);
                         if (w_T)
-                            store4(a.T_out + o, zero4);
+                            store4_out(a.T_out + o, zero4, a.stream_out);
                     } else {
                         for (int e = 0; e < 4 && i0 + e < nx; ++e) {
                             if (w_phi)
@@ -708,7 +721,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             if (r >= 1 && r <= TY && uy < ny && (fired[0] | fired[1] | fired[2] | fired[3])) {
                 R *dst = a.theta + (size_t)uy * nx + ux;
                 if (ux + 3 < nx && (nx & 3) == 0) {
-                    store4(dst, th);
+                    store4_out(dst, th, a.stream_out);
                 } else {
 #pragma unroll
                     for (int e = 0; e < 4; ++e)
@@ -882,8 +895,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     const size_t o = (size_t)j * nx + i0;
     if (i0 + 3 < nx && (nx & 3) == 0) {
         if (w_phi)
-            store4_out(a.phi_out + o, phin);
-        store4_out(a.T_out + o, Tn);
+            store4_out(a.phi_out + o, phin, a.stream_out);
+        store4_out(a.T_out + o, Tn, a.stream_out);
     } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {

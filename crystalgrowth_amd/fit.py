@@ -4,8 +4,8 @@ Each member m of an ensemble has hidden parameters; its target is phi after
 `steps` steps from a disk nucleus.  Starting from a common guess, every
 iteration resets all members to the same initial state (same noise counters),
 runs the batched forward + adjoint (kg_batch_adjoint) for
-L_m = sum (phi_m - target_m)^2, and takes one Adam step per member on the
-fitted parameters.  `looped_adjoint_seconds` times the same gradient
+L_m = sum (phi_m - target_m)^2, and takes one quasi-Newton (BFGS, with
+backtracking) or Adam step per member on the fitted parameters.  `looped_adjoint_seconds` times the same gradient
 evaluation done member by member with plain handles (kg_adjoint), the
 baseline the batched path is compared with.
 
@@ -43,17 +43,103 @@ def make_targets(sizes, hidden, steps, precision, r=5.0):
         return [b.phase(m) for m in range(len(sizes))]
 
 
+# Typical spread of each fitted parameter: quasi-Newton steps are taken in
+# the parameters divided by these.
+SCALE = {"delta": 0.02, "K": 0.4, "theta0": 0.5}
+
+
+def _evaluate(b, cur, targets, steps, every, r):
+    """Loss and gradient of every member at parameters cur (one batched
+    forward + adjoint from the common initial state)."""
+    for m, p in enumerate(cur):
+        b.set_params(m, p)
+    b.reset_disk(-1, -1, -1, r)
+    grads, _ = b.adjoint(steps, targets, every)
+    f = np.array([g["loss"] for g in grads])
+    g = np.array([[g["d_" + n] for n in NAMES] for g in grads])
+    return f, g
+
+
+def _clip(p, u):
+    upd = {}
+    for k, n in enumerate(NAMES):
+        lo, hi = BOUNDS[n]
+        upd[n] = float(min(max(u[k] * SCALE[n], lo), hi))
+    return p.replace(**upd)
+
+
+def _fit_bfgs(b, guess, targets, steps, iterations, every, r, log):
+    """Quasi-Newton (BFGS) per member with Armijo backtracking, all members
+    evaluated together: every iteration is one batched forward + adjoint at
+    each member's trial point.  A member accepts its trial when the loss
+    falls enough (Armijo, c1 = 1e-4) and updates its inverse-Hessian
+    estimate; otherwise it shortens the step (x 0.3) along the same
+    direction.  Works in the parameters divided by SCALE."""
+    M = len(guess)
+    sc = np.array([SCALE[n] for n in NAMES])
+    x = np.array([[getattr(p, n) for n in NAMES] for p in guess]) / sc
+    f, gx = _evaluate(b, [_clip(p, u) for p, u in zip(guess, x)], targets, steps, every, r)
+    g = gx * sc
+    losses = [float(f.sum())]
+    H = np.array([np.eye(3) * 0.2 / max(np.linalg.norm(gm), 1e-300) for gm in g])
+    d = -np.einsum("mij,mj->mi", H, g)
+    t = np.ones(M)
+    if log:
+        log(1, losses[-1])
+    for it in range(2, iterations + 1):
+        trial = x + t[:, None] * d
+        ft, gxt = _evaluate(b, [_clip(p, u) for p, u in zip(guess, trial)], targets, steps,
+                            every, r)
+        gt = gxt * sc
+        for m in range(M):
+            slope = float(g[m] @ d[m])
+            if ft[m] <= f[m] + 1e-4 * t[m] * slope:
+                s_, y_ = trial[m] - x[m], gt[m] - g[m]
+                sy = float(s_ @ y_)
+                if sy > 1e-12 * float(np.linalg.norm(s_) * np.linalg.norm(y_)):
+                    rho = 1.0 / sy
+                    V = np.eye(3) - rho * np.outer(s_, y_)
+                    H[m] = V @ H[m] @ V.T + rho * np.outer(s_, s_)
+                x[m], f[m], g[m] = trial[m], ft[m], gt[m]
+                d[m] = -H[m] @ g[m]
+                if float(g[m] @ d[m]) >= 0:  # not a descent direction: restart
+                    H[m] = np.eye(3) * 0.2 / max(np.linalg.norm(g[m]), 1e-300)
+                    d[m] = -H[m] @ g[m]
+                t[m] = 1.0
+            else:
+                t[m] *= 0.3
+        losses.append(float(f.sum()))
+        if log:
+            log(it, losses[-1])
+    return [_clip(p, u) for p, u in zip(guess, x)], losses
+
+
 def fit(sizes, hidden, guess, steps, iterations, every=50, precision="f32", r=5.0, log=None,
-        lr=None, schedule="constant"):
-    """Adam on (delta, K, theta0) of every member; `lr` maps each name to its
-    initial learning rate (default LR), `schedule` is "constant" or "cosine"."""
+        lr=None, schedule="constant", beta2=0.999, method="bfgs"):
+    """Fit (delta, K, theta0) of every member to its target.  method "bfgs"
+    (default): batched quasi-Newton with backtracking (_fit_bfgs); "adam":
+    Adam with per-parameter learning rates `lr` (default LR), `schedule`
+    "constant" or "cosine", second-moment decay `beta2`.  Each iteration is
+    one batched forward + adjoint."""
     lr = dict(LR if lr is None else lr)
     targets = make_targets(sizes, hidden, steps, precision, r)
+    if method == "bfgs":
+        t0 = time.perf_counter()
+        with Batch(sizes, guess, precision) as b:
+            cur, losses = _fit_bfgs(b, guess, targets, steps, iterations, every, r, log)
+        return {"members": len(sizes), "steps": steps, "iterations": iterations,
+                "method": method, "loss_first": losses[0], "loss_last": losses[-1],
+                "param_error_start": param_errors(guess, hidden),
+                "param_error_end": param_errors(cur, hidden),
+                "batched_adjoint_seconds": time.perf_counter() - t0,
+                "fitted": [{n: getattr(c, n) for n in NAMES} for c in cur]}
+    if method != "adam":
+        raise ValueError("method must be 'bfgs' or 'adam'")
     cur = [g.replace() for g in guess]
     M = len(sizes)
     m1 = np.zeros((M, len(NAMES)))
     m2 = np.zeros((M, len(NAMES)))
-    b1, b2, eps = 0.9, 0.999, 1e-12
+    b1, b2, eps = 0.9, beta2, 1e-12
     losses = []
     t_adj = 0.0
     with Batch(sizes, cur, precision) as b:
@@ -80,7 +166,8 @@ def fit(sizes, hidden, guess, steps, iterations, every=50, precision="f32", r=5.
                 cur[m] = cur[m].replace(**upd)
             if log:
                 log(it, losses[-1])
-    return {"members": M, "steps": steps, "iterations": iterations, "loss_first": losses[0],
+    return {"members": M, "steps": steps, "iterations": iterations, "method": method,
+            "loss_first": losses[0],
             "loss_last": losses[-1], "param_error_start": param_errors(guess, hidden),
             "param_error_end": param_errors(cur, hidden), "batched_adjoint_seconds": t_adj,
             "fitted": [{n: getattr(c, n) for n in NAMES} for c in cur]}

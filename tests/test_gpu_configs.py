@@ -9,6 +9,10 @@
       (relative L2 <= 1e-5 on phi and on T), and the same
       eight members with periodic boundaries; all 512 members bit for bit
       equal to plain handles on the dense kernel after 1000 steps.
+  c4  the 64-member batched fit (sides 128..512, 500 steps, FP32): the
+      batched gradients of the two smallest members and of a middle one
+      against central differences of the FP64 oracle, and two Adam
+      iterations lower the loss.
 The oracle runs on the host's cores (its OpenMP build, bitwise the serial
 oracle: tests/test_oracle_parallel.py).
 """
@@ -107,3 +111,52 @@ def test_c2_all_members_match_plain_dense_handles_bitwise(kb):
                 for name, x, y in zip(("phi", "T", "theta"), b.fields(m), s.fields()):
                     assert np.array_equal(x.view(np.uint32), y.view(np.uint32)), (m, nx, name)
     assert stats["listed_launches"] > 0, stats   # chunk lists took part
+
+
+def test_c4_batched_gradients_match_oracle_central_differences(kb, O):
+    """dL/d(delta, K, theta0) of kg_batch_adjoint over the whole 64-member
+    ensemble, FP32 and FP64, against central differences of the FP64 oracle
+    with h = 1e-6 (relative for delta and K, absolute for theta0).  The loss
+    is piecewise smooth: the angle update switches branch where a gradient
+    crosses FLT_EPSILON (src/Kobayashi.cpp:154-167), and with h = 1e-4 some
+    of those switches fall inside the difference interval (measured on
+    MI355X: the FP64 adjoint and central differences then differ by up to
+    1.3 % for K); with h = 1e-6 they agree to <= 6e-6.  Tolerances: FP64
+    adjoint <= 1e-4, FP32 adjoint <= 1e-3 relative."""
+    from crystalgrowth_amd import configs, fit
+    c = configs.C4
+    sizes, hidden, guess = configs.c4_members(c["count"])
+    steps, every = c["steps"], c["every"]
+    targets = fit.make_targets(sizes, hidden, steps, "f32")
+    grads = {}
+    for prec in ("f32", "f64"):
+        with kb.Batch(sizes, guess, prec) as b:
+            b.reset_disk(-1, -1, -1, 5.0)
+            grads[prec], _ = b.adjoint(steps, [t.astype(b.dtype) for t in targets], every)
+    order = sorted(range(len(sizes)), key=lambda m: (sizes[m][0], m))
+    for m in order[:2] + [order[len(order) // 2]]:
+        nx, ny = sizes[m]
+        tgt = targets[m].astype(np.float64)
+
+        def loss(p):
+            s = O.run_parallel(O.init_disk(nx, ny, nx // 2, ny // 2, 5.0, np.float64),
+                               oracle_params(O, p), steps)
+            d = s.phi - tgt
+            return float((d * d).sum())
+
+        for name in ("delta", "K", "theta0"):
+            v = getattr(guess[m], name)
+            h = 1e-6 if name == "theta0" else 1e-6 * abs(v)
+            ref = (loss(guess[m].replace(**{name: v + h})) -
+                   loss(guess[m].replace(**{name: v - h}))) / (2 * h)
+            g64, g32 = grads["f64"][m]["d_" + name], grads["f32"][m]["d_" + name]
+            assert abs(g64 - ref) <= 1e-4 * abs(ref), (m, nx, name, g64, ref)
+            assert abs(g32 - ref) <= 1e-3 * abs(ref), (m, nx, name, g32, ref)
+
+
+def test_c4_fit_iterations_lower_the_loss(kb):
+    from crystalgrowth_amd import configs, fit
+    c = configs.C4
+    sizes, hidden, guess = configs.c4_members(c["count"])
+    res = fit.fit(sizes, hidden, guess, c["steps"], 2, c["every"])
+    assert res["members"] == 64 and res["loss_last"] < res["loss_first"], res
