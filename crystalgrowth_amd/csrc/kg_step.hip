@@ -641,6 +641,28 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     }
     if (T_known && !spec_T)
         fill_rows(sT, QH, (R)0);
+    // The old theta of the pass-1 cells: staged in F with phi and T for a
+    // tile active last step; otherwise each pass-1 thread loads its own four
+    // angles (the 16-byte segment staging would put in its slots) and each
+    // halo-column thread its one, straight into registers, waited on only
+    // where pass 1 first needs an old angle.  A wave that has seen a
+    // non-flat phi knows the tile is non-flat and issues them before the
+    // flatness barrier; the other waves after it, if the tile is non-flat.
+    // Widths not a multiple of 4 stage cell by cell into F.
+    const bool theta_direct = !active && (nx & 3) == 0;
+    R thold[4] = {(R)0, (R)0, (R)0, (R)0}, thalo = (R)0;
+    int thedge = 0;  // zero-flux edge code of the segment, applied where it is used
+    bool theta_issued = false;
+    auto issue_theta = [&]() {
+        const size_t o = segment_offset<BC>(nx, ny, x0, y0 - 1,
+                                            (tid / NG) * (LW / 4) + tid % NG + 1, thedge);
+        load4(a.theta + o, thold);
+        if (tid < 2 * QH) {
+            const int r = tid >> 1, ux = (tid & 1) ? x0 + TX : x0 - 1;
+            thalo = a.theta[(size_t)map_index<BC>(y0 - 1 + r, ny) * nx + map_index<BC>(ux, nx)];
+        }
+        theta_issued = true;
+    };
     // block-wide AND of the two flatness flags with one barrier: each wave
     // posts its ballots, every thread combines the eight
     {
@@ -648,6 +670,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         const bool w1 = __ballot(all1 & finite) == __ballot(1);
         if ((tid & 63) == 0)
             sm.W[tid >> 6] = (w0 ? 1 : 0) | (w1 ? 2 : 0);
+        if (theta_direct && !w0 && !w1)
+            issue_theta();
     }
     __syncthreads();
     int fl = 3;
@@ -658,17 +682,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     const bool flat = fl != 0;
 
     if (!flat) {
-        // ---- stage the old theta of the pass-1 cells into F ---------------
-        if (!active) {
+        if (!active && (nx & 3) != 0) {
             int unused0 = 1, unused1 = 1;
-            if ((nx & 3) == 0) {
-                Seg2<R> gh;
-                issue_rows_w<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
-                commit_rows<R, QH, 0>(gh, sF, unused0, unused1);
-            } else {
-                stage_rows<R, BC, 0>(a.theta, sF, nx, ny, x0, y0 - 1, QH, unused0, unused1);
-            }
+            stage_rows<R, BC, 0>(a.theta, sF, nx, ny, x0, y0 - 1, QH, unused0, unused1);
         }
+        if (theta_direct && !theta_issued)
+            issue_theta();
         if (phi_known)
             fill_rows(sP, PH, (known & S_PHI0) ? (R)0 : (R)1);
         if (tid == 0) {
@@ -685,9 +704,10 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             const int g = tid % NG, r = tid / NG;       // pass-1 row r: global y0-1+r
             const int kc = 4 * g + 4;                   // LDS column of the first cell
             const int ux = x0 + 4 * g, uy = y0 - 1 + r;
-            R e2[4], F[4], th[4], thold[4];
+            R e2[4], F[4], th[4];
             bool fired[4];
-            load4(&sF[r * LW + kc], thold);
+            if (!theta_direct)
+                load4(&sF[r * LW + kc], thold);
             // zero-flux ghost rows/columns take the values of the cell they
             // clamp to; periodic halos already hold wrapped values
             const bool shifted = BC == 1 && (uy < 0 || uy >= ny || ux + 3 >= nx);
@@ -699,6 +719,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
                 load4(pm + kc, m + 1);
                 load4(pp + kc, p + 1);
             }
+            if (theta_direct)
+                fix_edge(thold, thedge);
             if (!shifted) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e)
@@ -745,7 +767,7 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             R e2, F, th;
             bool fired;
             pass1_cell(a, z0, &sP[lr * LW], &sP[(lr + 1) * LW], &sP[(lr + 2) * LW], lc,
-                       sF[r * LW + lcol], e2, F, th, fired);
+                       theta_direct ? thalo : sF[r * LW + lcol], e2, F, th, fired);
             sE2[r * LW + lcol] = e2;
             sF[r * LW + lcol] = F;
         }

@@ -317,3 +317,38 @@ def test_colormap_rgba8_is_unorm8_of_float_ramp(kb):
             assert b.dtype == np.uint8 and b.shape == (n * n, 4)
             assert np.array_equal(b[:, :3], want.astype(np.uint8))
             assert (b[:, 3] == 255).all()
+
+
+@pytest.mark.parametrize("case", ["c1_256", "c0_300"])
+def test_fp32_per_cell_to_2000_steps(kb, O, case):
+    """FP32 per cell beyond the short horizons, from a disk at the centre
+    (zero-flux, noise): after 1000 steps the GPU is within 1e-5 relative L2
+    of the FP32 oracle and of the FP64 oracle (measured on MI355X: <= 2.9e-6);
+    after 2000 steps dendritic growth has amplified rounding differences
+    (the FP32 oracle itself is 1.3e-4 from the FP64 oracle on 256^2), and
+    the GPU's FP32 stays within 3x of the FP32 oracle's own distance to FP64
+    (measured: 1.3x and 0.8x)."""
+    n, phys = {"c1_256": (256, dict(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                                    boundary=1, seed=3)),
+               "c0_300": (300, dict(delta=0.02, aniso=4.0, K=1.6, theta0=0.2, noise_a=0.01,
+                                    boundary=1, seed=0))}[case]
+    p = O.default_params(**phys)
+    s32 = O.init_disk(n, n, n // 2, n // 2, 5.0, FLOAT)
+    s64 = O.init_disk(n, n, n // 2, n // 2, 5.0, DOUBLE)
+    with kb.Solver(n, n, "f32", gpu_params(kb, p)) as g:
+        g.reset_disk(n // 2, n // 2, 5.0)
+        done = 0
+        for mark in (1000, 2000):
+            s32 = O.run_parallel(s32, p, mark - done, step0=done)
+            s64 = O.run_parallel(s64, p, mark - done, step0=done)
+            g.step(mark - done)
+            done = mark
+            phi, T, _ = g.fields()
+            if mark == 1000:
+                for a, b32, b64 in ((phi, s32.phi, s64.phi), (T, s32.T, s64.T)):
+                    assert rel(a, b32.astype(DOUBLE)) <= 1e-5
+                    assert rel(a, b64) <= 1e-5
+            else:
+                own = rel(s32.phi, s64.phi)
+                assert rel(phi, s64.phi) <= 3 * own, (rel(phi, s64.phi), own)
+                assert rel(T, s64.T) <= 3 * rel(s32.T, s64.T)
