@@ -613,6 +613,18 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
     int all0 = (known & S_PHI0) ? 1 : 0, all1 = (known & S_PHI1) ? 1 : 0, finite = 1;
     if (!phi_known)
         all0 = all1 = 1;
+    // eps and eps eps' at theta = +0 (see ZeroAngle), computed by thread 0
+    // into LDS.  Without the status it is done while the staging loads are
+    // in flight, and the flatness barrier publishes it; with the status (whose
+    // quiescent tiles return before any of this) only for a non-flat tile,
+    // behind a barrier of its own.
+    const bool z_early = !track;
+    auto zero_angle = [&]() {
+        R e0, f0;
+        eps_of_angle(a, (R)0, e0, f0);
+        sm.Z[0] = e0;
+        sm.Z[1] = f0;
+    };
     if ((nx & 3) == 0) {
         Seg2<R> gp, gh;
         if (!phi_known)
@@ -621,6 +633,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             issue_rows_w<R, BC, QH>(a.T_in, nx, ny, x0, y0 - 1, gt);
         if (active)
             issue_rows_w<R, BC, QH>(a.theta, nx, ny, x0, y0 - 1, gh);
+        if (z_early && tid == 0)
+            zero_angle();
         int unused0 = 1, unused1 = 1;
         if (!phi_known)
             commit_rows<R, PH, 1>(gp, sP, all0, all1);
@@ -630,6 +644,8 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
         if (active)
             commit_rows<R, QH, 0>(gh, sF, unused0, unused1);
     } else {
+        if (z_early && tid == 0)
+            zero_angle();
         if (!phi_known)
             stage_rows<R, BC, 1>(a.phi_in, sP, nx, ny, x0, y0 - 2, PH, all0, all1);
         if (!T_known)
@@ -690,13 +706,12 @@ __device__ __forceinline__ void step_tile(const StepArgs<R> &a, int x0, int y0, 
             issue_theta();
         if (phi_known)
             fill_rows(sP, PH, (known & S_PHI0) ? (R)0 : (R)1);
-        if (tid == 0) {
-            R e0, f0;
-            eps_of_angle(a, (R)0, e0, f0);
-            sm.Z[0] = e0;
-            sm.Z[1] = f0;
-        }
-        __syncthreads();
+        if (!z_early && tid == 0)
+            zero_angle();
+        // LDS written since the flatness barrier: the zero angle (status
+        // kernels), phi rows of a known constant, theta staged cell by cell
+        if (!z_early || phi_known || (!active && (nx & 3) != 0))
+            __syncthreads();
         const ZeroAngle<R> z0{sm.Z[0], sm.Z[1]};
 
         // ---- pass 1: one 4-cell group per thread (columns x0 .. x0+63) ----
