@@ -142,6 +142,22 @@ int skip_cycle(kg_solver_s &h, int &slot)
         else if (h.live_order.size() <= (size_t)kg::LIVE_MAX) {
             std::copy(h.live_order.begin(), h.live_order.end(), h.live.c);
             h.live.n = (int)h.live_order.size();
+            h.live.dev = nullptr;
+        } else if (h.live_dev[0]) {
+            // longer than the kernel arguments hold: through device memory.
+            // Launches still reading this buffer were queued two periods
+            // ago, before the copy; its staging buffer is free once the
+            // copy queued then has run.
+            const int b = h.live_dev_next;
+            h.live_dev_next ^= 1;
+            KG_HIP(hipEventSynchronize(h.live_copy_ev[b]));
+            std::copy(h.live_order.begin(), h.live_order.end(), h.live_stage[b]);
+            KG_HIP(hipMemcpyAsync(h.live_dev[b], h.live_stage[b],
+                                  h.live_order.size() * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                  h.stream));
+            KG_HIP(hipEventRecord(h.live_copy_ev[b], h.stream));
+            h.live.n = (int)h.live_order.size();
+            h.live.dev = reinterpret_cast<const uint32_t *>(h.live_dev[b]);
         }
         h.live_pending[p] = false;
     }
@@ -257,6 +273,14 @@ void free_all(kg_solver_s *h)
             (void)hipHostFree(h->live_host[p]);
         if (h->live_ev[p])
             (void)hipEventDestroy(h->live_ev[p]);
+    }
+    for (int b = 0; b < 2; ++b) {
+        if (h->live_dev[b])
+            (void)hipFree(h->live_dev[b]);
+        if (h->live_stage[b])
+            (void)hipHostFree(h->live_stage[b]);
+        if (h->live_copy_ev[b])
+            (void)hipEventDestroy(h->live_copy_ev[b]);
     }
     if (h->ev0)
         (void)hipEventDestroy(h->ev0);
@@ -452,6 +476,20 @@ int kg_create(int nx, int ny, int precision, const kg_params *p, kg_handle *out)
                 ok = ok && hipEventCreateWithFlags(&h->live_ev[q],
                                                    hipEventDisableTiming |
                                                        hipEventDisableSystemFence) == hipSuccess;
+            }
+            // lists longer than the kernel arguments hold go through device
+            // memory (two buffers used in turn, 4-byte padded)
+            const size_t lb = ((size_t)nc + 1) / 2 * 4;
+            for (int b = 0; b < 2 && ok && nc > kg::LIVE_MAX; ++b) {
+                ok = hipMalloc(reinterpret_cast<void **>(&h->live_dev[b]), lb) == hipSuccess;
+                if (!ok)
+                    h->live_dev[b] = nullptr;
+                ok = ok && hipHostMalloc(reinterpret_cast<void **>(&h->live_stage[b]), lb,
+                                         hipHostMallocDefault) == hipSuccess;
+                if (!ok)
+                    h->live_stage[b] = nullptr;
+                ok = ok && hipEventCreateWithFlags(&h->live_copy_ev[b], hipEventDisableTiming) ==
+                               hipSuccess;
             }
             if (!ok) {
                 free_all(h);

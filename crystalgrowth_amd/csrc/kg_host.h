@@ -44,7 +44,7 @@ struct kg_solver_s {
     // use (n < 0: all), and SKIP_AGE pinned host buffers of liveness flags,
     // each with the event of the kernel that writes it, the tag that request
     // writes and whether it is still to be read
-    kg::LiveList live{-1, {}};
+    kg::LiveList live{-1, nullptr, {}};
     std::vector<uint16_t> live_order;
     int nchunks = 0;
     uint8_t *live_host[kg::SKIP_AGE] = {};
@@ -52,6 +52,13 @@ struct kg_solver_s {
     uint32_t live_tag[kg::SKIP_AGE] = {};
     bool live_pending[kg::SKIP_AGE] = {};
     uint32_t next_tag = 0;
+    // lists longer than LIVE_MAX: two device copies used in turn, each
+    // filled from a pinned staging buffer by an asynchronous copy whose
+    // event guards the staging buffer's next use
+    uint16_t *live_dev[2] = {};
+    uint16_t *live_stage[2] = {};
+    hipEvent_t live_copy_ev[2] = {};
+    int live_dev_next = 0;
     uint64_t live_stale = 0;       // answers found not yet written (full launches instead)
     uint64_t listed_launches = 0;  // step launches that covered only listed chunks
     // bound on |theta| of angles not written by the step (resets write 0,

@@ -67,11 +67,14 @@ struct ReduceOut {
 // cx) in the order the launch deals them to the XCDs.  Every chunk left out
 // holds phi = +0 and T = +0 on both buffers and keeps them over the launch,
 // and its status words already say so (the host derives the list from
-// a liveness request; kg_api.hip states the conditions).  Passed by value in
-// the kernel arguments: the launch has only the listed chunks' blocks.
+// a liveness request; kg_api.hip states the conditions).  Up to LIVE_MAX
+// entries travel by value in the kernel arguments; a longer list is copied
+// to device memory first and `dev` points at it (the same 16-bit entries).
+// Either way the launch has only the listed chunks' blocks.
 constexpr int LIVE_MAX = 1020;
 struct LiveList {
     int n;
+    const uint32_t *dev;              // n > LIVE_MAX: the entries, in device memory
     alignas(4) uint16_t c[LIVE_MAX];  // read by the kernel as 32-bit words
 };
 static_assert(LIVE_MAX % 2 == 0, "LiveList entries are read in pairs");

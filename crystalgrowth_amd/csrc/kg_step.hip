@@ -1001,7 +1001,8 @@ __device__ __forceinline__ bool listed_tile(const LiveList &live, int b, int til
     if (j >= live.n)
         return false;
     // the 32-bit word holding entry j (a scalar load; there is no 16-bit one)
-    const uint32_t w = reinterpret_cast<const uint32_t *>(live.c)[j >> 1];
+    const uint32_t w =
+        live.dev ? live.dev[j >> 1] : reinterpret_cast<const uint32_t *>(live.c)[j >> 1];
     const int c = (int)((j & 1) ? w >> 16 : w & 0xffffu);
     const int k = i % (CHUNK_W * CHUNK_H);
     const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;

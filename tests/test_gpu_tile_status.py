@@ -225,3 +225,34 @@ def test_chunk_skipping_across_step_count_changes(kb):
                 s.step(n)
         assert_bitwise([(121, a.fields(), b.fields())])
         assert a.skip_stats()["listed_launches"] > 0
+
+
+def test_chunk_lists_longer_than_kernel_arguments(kb):
+    """More than 1020 live chunks (the kernel arguments' list capacity): the
+    list goes to the launch through device memory, and the run must still
+    match the dense kernel bitwise.  6144 x 4096 cells are 48 x 35 chunks
+    (1680); 800 nuclei spread over the grid keep about 1280 of them live
+    from the start."""
+    nx, ny = 6144, 4096
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                          boundary=1, seed=9)
+    with kb.Solver(nx, ny, "f32", p) as s:
+        s.reset_seeds(800, 5.0, 8, 21)
+        phi0 = s.phase()
+    # live chunks of the initial state: a chunk (2 x 4 tiles of 64 x 30) is
+    # live when a tile within one tile column / row of it holds phi != 0
+    tiles_y, tiles_x = -(-ny // 30), -(-nx // 64)
+    busy = np.zeros((tiles_y, tiles_x), bool)
+    ys, xs = np.nonzero(phi0)
+    busy[ys // 30, xs // 64] = True
+    live = 0
+    for cy in range(-(-tiles_y // 4)):
+        for cx in range(tiles_x // 2):
+            y0, x0 = max(cy * 4 - 1, 0), max(cx * 2 - 1, 0)
+            live += bool(busy[y0:cy * 4 + 5, x0:cx * 2 + 3].any())
+    assert 1020 < live < 1680, live
+    stats = []
+    pairs = run_pair(kb, nx, ny, "f32", p, lambda s: s.reset_seeds(800, 5.0, 8, 21),
+                     [9, 50], stats)
+    assert_bitwise(pairs)
+    assert stats[0]["listed_launches"] >= 30 and stats[0]["stale_answers"] == 0, stats
