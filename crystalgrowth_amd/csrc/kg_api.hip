@@ -589,6 +589,14 @@ int kg_step(kg_handle h, int nsteps)
     if (nsteps < 0)
         return fail(KG_ERR_INVALID_ARGUMENT, "nsteps must be >= 0");
     KG_HIP(hipSetDevice(h->device));
+    // a captured launch would replay with the step count (noise counter) and
+    // buffer parity it was captured with: refuse rather than record it
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    KG_HIP(hipStreamIsCapturing(h->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return fail(KG_ERR_INVALID_ARGUMENT,
+                    "kg_step cannot be captured into a graph (each launch carries its step "
+                    "count and buffer parity); a batch in KG_BATCH_GRAPH mode replays correctly");
     if (h->kernel == KG_KERNEL_TWO_PASS)
         if (int rc = ensure_temps(*h))
             return rc;

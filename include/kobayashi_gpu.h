@@ -126,7 +126,9 @@ int kg_get_size(kg_handle h, int *nx, int *ny, int *precision);
  * (src/Kobayashi.cpp:227-239).  Asynchronous on the handle's stream; with
  * the default kernel the host stays at most about 14 steps ahead of the GPU
  * (every 7 steps it waits for the GPU's answer on which regions stay all
- * +0). */
+ * +0).  KG_ERR_INVALID_ARGUMENT while the handle's stream is being captured
+ * into a graph (a replay would repeat the captured step; batches in
+ * KG_BATCH_GRAPH mode replay correctly). */
 int kg_step(kg_handle h, int nsteps);
 /* kg_step bracketed by HIP events on the handle's stream; *ms = elapsed
  * milliseconds of the nsteps launches (waits for completion). */
