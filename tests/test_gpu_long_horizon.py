@@ -82,3 +82,27 @@ def test_c1_physics_morphology_tip_velocity(kb, O):
     # the device's morphology of the final state agrees with the host's
     assert dev["arm_count"] == 6
     assert dev["tip_radius"] == pytest.approx(M.tip_radius(gb, c, c), abs=0.3)
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", 2e-3), ("f64", 1e-11)])
+def test_headline_invariant_at_full_size(kb, precision, tol):
+    """The bench's 4096 x 4096 workload over the first 10000 steps of its
+    horizon: sum(T - K phi) is conserved by the model (the 9-point stencil's
+    weights telescope and the noise enters phi and T together), so it may
+    move only by accumulated rounding (measured: 4.1e-4 of its value in
+    FP32, 3.4e-14 in FP64), while a dropped or mis-signed term of either update, or a K off
+    by 1 %, moves it by more than its whole value (sum(phi) grows from 729
+    to about 5.7e5 over these steps).  The fields stay finite and the
+    crystals grow."""
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01,
+                          boundary=1, seed=0)
+    with kb.Solver(4096, 4096, precision, p) as s:
+        s.reset_seeds(9, 5.0, 8, 0)
+        st0 = s.stats()
+        s.step(10000)
+        st = s.stats()
+    inv0, inv = st0["sum_T_minus_K_phi"], st["sum_T_minus_K_phi"]
+    print("sum(T - K phi):", inv0, "->", inv, "relative drift", abs(inv - inv0) / abs(inv0))
+    assert st["nonfinite"] == 0
+    assert st["solid_fraction"] > 5 * st0["solid_fraction"]
+    assert abs(inv - inv0) <= tol * abs(inv0)
