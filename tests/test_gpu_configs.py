@@ -6,7 +6,7 @@
   c2  the 512-member ensemble (sides 128..1024, per-member delta, K, j,
       theta0, eps_bar, zero-flux): eight members spanning the sides and all
       three j, after 100 batched steps, against the FP32 and the FP64 oracle
-      (relative L2 <= 1e-5 on phi and on T), and the same
+      (relative L2 <= 1e-5 on phi and on T, the FP32 accuracy target), and the same
       eight members with periodic boundaries; all 512 members bit for bit
       equal to plain handles on the dense kernel after 1000 steps.
   c4  the 64-member batched fit (sides 128..512, 500 steps, FP32): the

@@ -8,7 +8,7 @@ Tolerances (written here, per the precision of the path):
       values; device atanf/sincosf, reciprocal multiplies and FMA differ from
       glibc's float functions and the reference's divisions by ulps).
   FP32 kernel vs FP64 oracle at 100 steps on the disk-seeded configurations:
-      relative L2 <= 1e-5 on phi and on T (the north-star accuracy bar).
+      relative L2 <= 1e-5 on phi and on T (the FP32 accuracy target).
 The plus-shaped reference nucleus is compared per cell only for 4 steps:
 its centre has an exactly zero gradient, and from step 5 rounding noise
 decides whether theta fires there (src/Kobayashi.cpp:154-167), in any two
@@ -137,7 +137,7 @@ def test_fp32_plus_nucleus_four_steps(kb, O):
 
 @pytest.mark.parametrize("cfg", ["c0", "c1"])
 def test_fp32_within_1e5_of_fp64_at_100_steps(kb, O, cfg):
-    """North-star accuracy: FP32 step within 1e-5 relative (L2) of FP64 after
+    """FP32 accuracy target: FP32 step within 1e-5 relative (L2) of FP64 after
     100 steps, on the configurations' own setup (disk seeds, zero-flux, noise)."""
     if cfg == "c0":
         kw = dict(delta=0.02, aniso=4.0, theta0=0.2, noise_a=0.01, boundary=1)
