@@ -32,9 +32,16 @@ Beside the headline, labelled apart:
                   plain solver per member (one handle and stream per member)
   adjoint      -- loss + gradients of 1000 steps on 512^2 (checkpoint every 50)
                   vs the plain forward of the same steps
-  fit          -- the 64-member batched parameter fit (500 steps, 50 Adam
-                  iterations): recovered-parameter errors, and one batched
-                  gradient against the same gradient looped over plain handles
+  fit          -- the 64-member batched parameter fit (500 steps, 50
+                  batched-BFGS iterations): recovered-parameter errors, and one
+                  batched gradient against the same gradient looped over plain
+                  handles
+  ceilings     -- what bounds the headline on this GPU: a device copy of the
+                  dense step's algorithmic bytes (read 2 x 64 MiB, write
+                  2 x 64 MiB; four buffer pairs in turn, so that sources are
+                  not served from the last-level cache) and the dense launch
+                  on the same grid with every tile flat (all fields +0), both
+                  priced like the headline
   reference_config -- the reference's configuration (300^2 FP64, 2000 steps)
                   on the GPU and on the CPU oracle, compared cell by cell
   cpu_all_cores -- the oracle on OpenMP threads
@@ -253,6 +260,77 @@ def calibrate_copy():
                       "bytes_written_per_copy": 4 * n, "copies": 20}), flush=True)
 
 
+def ceilings(kb, size: int, launches: int = 50):
+    """Two references for the headline's roofline fraction, measured on the
+    same GPU: a device-to-device copy of phi and T (the dense step's
+    algorithmic bytes: 2 fields read, 2 written) and the dense launch on an
+    all-+0 grid, where every tile takes the flat path (one memory round trip,
+    no pass 1).  Both priced at 16 B per cell like the headline."""
+    import ctypes
+    from crystalgrowth_amd._lib import lib
+    hip = lib()  # the HIP runtime the solver itself uses
+    vp = ctypes.c_void_p
+    for name, args in (("hipMalloc", [ctypes.POINTER(vp), ctypes.c_size_t]), ("hipFree", [vp]),
+                       ("hipMemcpyAsync", [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]),
+                       ("hipEventCreate", [ctypes.POINTER(vp)]), ("hipEventDestroy", [vp]),
+                       ("hipEventRecord", [vp, vp]), ("hipEventSynchronize", [vp]),
+                       ("hipEventElapsedTime", [ctypes.POINTER(ctypes.c_float), vp, vp])):
+        getattr(hip, name).argtypes = args
+    cells = size * size
+    alg = ALG_BYTES_FP32 * cells
+    nbytes = 2 * 4 * cells  # phi and T, FP32
+    # four source/destination pairs used in turn (1 GiB in all), so that a
+    # copy's source is not still in the 256 MiB last-level cache from the
+    # copy before it, as the dense step's inputs are not
+    pairs = [(vp(), vp()) for _ in range(4)]
+    e0, e1 = vp(), vp()
+
+    def ok(rc):
+        if rc != 0:
+            raise RuntimeError(f"bench: HIP error {rc} in the copy ceiling")
+
+    try:
+        for src, dst in pairs:
+            ok(hip.hipMalloc(ctypes.byref(src), nbytes))
+            ok(hip.hipMalloc(ctypes.byref(dst), nbytes))
+        ok(hip.hipEventCreate(ctypes.byref(e0)))
+        ok(hip.hipEventCreate(ctypes.byref(e1)))
+        for src, dst in pairs:
+            ok(hip.hipMemcpyAsync(dst, src, nbytes, 3, None))  # hipMemcpyDeviceToDevice
+        ok(hip.hipEventRecord(e0, None))
+        for i in range(launches):
+            src, dst = pairs[i % len(pairs)]
+            ok(hip.hipMemcpyAsync(dst, src, nbytes, 3, None))
+        ok(hip.hipEventRecord(e1, None))
+        ok(hip.hipEventSynchronize(e1))
+        ms = ctypes.c_float(0.0)
+        ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1))
+        copy_us = ms.value * 1e3 / launches
+    finally:
+        for src, dst in pairs:
+            for p in (src, dst):
+                if p.value:
+                    hip.hipFree(p)
+        for e in (e0, e1):
+            if e.value:
+                hip.hipEventDestroy(e)
+    with kb.Solver(size, size, "f32", kb.default_params(**WORKLOAD)) as s:
+        s.set_kernel("fused_dense")
+        z = np.zeros((size, size), np.float32)
+        s.set_fields(z, z, z)
+        s.step(5)
+        flat_us = s.step_timed(launches) * 1e3 / launches
+    out = {}
+    for key, us in (("device_copy", copy_us), ("dense_all_flat", flat_us)):
+        out[key] = {"us": round(us, 3), "alg_GBs": round(alg / (us * 1e-6) / 1e9, 1),
+                    "frac_of_hbm_peak": round(alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)}
+    out["note"] = ("device_copy: hipMemcpyAsync device to device of 2 x 64 MiB, four buffer "
+                   "pairs in turn (HIP events); "
+                   "dense_all_flat: k_step_dense on a 4096^2 grid of +0 fields (HIP events on "
+                   "the solver's stream)")
+    return out
+
+
 def batched_vs_looped(kb, nsteps: int, looped: bool = True):
     """The ensemble from its initial state (disk r = 5 at every member's
     centre) for nsteps steps: one batched launch per step vs each member's
@@ -446,6 +524,7 @@ def main():
     if args.headline_only:
         print(json.dumps(rec), flush=True)
         return
+    rec["ceilings"] = ceilings(kb, n)
     if args.horizon > 0:
         rec["trajectory"] = trajectory(kb, n, args.horizon)
     if not args.no_fp64_check:
