@@ -278,16 +278,20 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
 // reuse between neighbouring chunks.  With launches of the live chunks only
 // (below) the smaller chunk skips more: 40.7 us at 2 x 8, 40.1 us at 2 x 4.
 constexpr int CHUNK_W = 2, CHUNK_H = 4, CHUNK_SKEW = 3;
-// The dense launch (no status, so no chunk lists) deals larger chunks: a
+// The dense launch (no status, so no chunk lists) deals wider chunks: a
 // tile row reads a 16-byte segment on each side of its 64 columns, i.e. a
 // 128-byte line of each neighbour, so a chunk's left and right halo lines
 // add 2 lines per chunk row to its own 2 per tile; wider chunks share more
-// of those lines in one XCD's L2.
+// of those lines in one XCD's L2.  One tile row of 8 tiles (512 x 30 cells)
+// measured best at 4096^2 (means of two runs, us per launch at steps 2000 /
+// 10000 / 19000): 8 x 1 48.3 / 52.0 / 59.6, 8 x 2 48.5 / 52.2 / 59.6,
+// 8 x 3 48.9 / 52.3 / 60.2, 8 x 4 48.7 / 53.0 / 60.3, 4 x 2 .. 4 x 4 as
+// 8 x 4, 16 x 1 49.1 / 55.5 / 65.7.
 #ifndef KG_DENSE_CHUNK_W
 #define KG_DENSE_CHUNK_W 8
 #endif
 #ifndef KG_DENSE_CHUNK_H
-#define KG_DENSE_CHUNK_H 4
+#define KG_DENSE_CHUNK_H 1
 #endif
 constexpr int DENSE_CHUNK_W = KG_DENSE_CHUNK_W, DENSE_CHUNK_H = KG_DENSE_CHUNK_H;
 
