@@ -163,6 +163,12 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 // Output tile size of the fused kernel (members of a batch are cut into it).
 int fused_tile_x();
 int fused_tile_y();
+// Chunk shape of the fused step's chunk lists (tiles) and the rotation of
+// successive chunk rows; a batch cuts its members into the same chunks,
+// which its liveness requests check.
+int fused_chunk_w();
+int fused_chunk_h();
+int fused_chunk_skew();
 
 // Batched step over a tile table (member, x0, y0); the step counter is
 // *step_base + step_add and its parity picks the ping-pong direction.

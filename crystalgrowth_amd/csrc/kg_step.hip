@@ -277,7 +277,13 @@ __device__ __forceinline__ void win6(const double *rowp, int kc, double w[6])
 // (2 x 4 the same); permuting the chunk order (49.7 - 51.5 us) loses the
 // reuse between neighbouring chunks.  With launches of the live chunks only
 // (below) the smaller chunk skips more: 40.7 us at 2 x 8, 40.1 us at 2 x 4.
-constexpr int CHUNK_W = 2, CHUNK_H = 4, CHUNK_SKEW = 3;
+#ifndef KG_CHUNK_W
+#define KG_CHUNK_W 2
+#endif
+#ifndef KG_CHUNK_H
+#define KG_CHUNK_H 4
+#endif
+constexpr int CHUNK_W = KG_CHUNK_W, CHUNK_H = KG_CHUNK_H, CHUNK_SKEW = 3;
 // The dense launch (no status, so no chunk lists) deals wider chunks: a
 // tile row reads a 16-byte segment on each side of its 64 columns, i.e. a
 // 128-byte line of each neighbour, so a chunk's left and right halo lines
@@ -1581,6 +1587,9 @@ int fused_chunks_y(int ny) { return ((ny + TY - 1) / TY + CHUNK_H - 1) / CHUNK_H
 
 int fused_tile_x() { return TX; }
 int fused_tile_y() { return TY; }
+int fused_chunk_w() { return CHUNK_W; }
+int fused_chunk_h() { return CHUNK_H; }
+int fused_chunk_skew() { return CHUNK_SKEW; }
 
 bool fused_status_ok(int nx, int ny, int boundary)
 {
