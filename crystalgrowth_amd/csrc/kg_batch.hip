@@ -77,15 +77,16 @@ void free_batch(kg_batch_s *b)
 }
 
 // The tile table, in block order: block b steps the tile of entry b.  Every
-// member is cut into the single-grid kernel's chunks (2 x 4 tiles, chunk
-// rows rotated by 3; the shape comes from kg_step.hip, whose liveness
-// requests check the same chunks); the chunks, member after member, are
+// member is cut into chunks of 2 x 2 tiles (chunk rows rotated by 3, as the
+// single-grid kernel orders its chunks; the shape comes from kg_step.hip,
+// whose batch liveness requests check the same chunks); the chunks, member
+// after member, are
 // dealt round-robin to the 8 XCDs (block b runs on XCD b % 8), each XCD
 // walks its chunks one after the other, and its sequence is padded to the
 // longest one with empty entries (member -1).  So every XCD steps tiles of
 // many members at any time, and tiles that share halos meet in one L2.
 constexpr int BNX = 8;
-const int BCW = kg::fused_chunk_w(), BCH = kg::fused_chunk_h(), BSKEW = kg::fused_chunk_skew();
+const int BCW = kg::batch_chunk_w(), BCH = kg::batch_chunk_h(), BSKEW = kg::fused_chunk_skew();
 
 // the chunks of every member in that order: (member, chunk column, chunk row)
 std::vector<int4> batch_chunks(const int *nx, const int *ny, int count)

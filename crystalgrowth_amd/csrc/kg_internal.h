@@ -164,11 +164,13 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 int fused_tile_x();
 int fused_tile_y();
 // Chunk shape of the fused step's chunk lists (tiles) and the rotation of
-// successive chunk rows; a batch cuts its members into the same chunks,
-// which its liveness requests check.
+// successive chunk rows; a batch cuts its members into chunks of
+// batch_chunk_w x batch_chunk_h tiles, which its liveness requests check.
 int fused_chunk_w();
 int fused_chunk_h();
 int fused_chunk_skew();
+int batch_chunk_w();
+int batch_chunk_h();
 
 // Batched step over a tile table (member, x0, y0); the step counter is
 // *step_base + step_add and its parity picks the ping-pong direction.
