@@ -123,7 +123,8 @@ def traffic_from_profiles(size: int):
 
 def cpu_baseline(state, nsteps: int, start: int):
     """The CPU oracle (one thread) for nsteps steps from the state the GPU
-    reached at trajectory step `start` (the headline window's first state)."""
+    reached at trajectory step `start` (the state right after the headline
+    window: reading the fields back before the timed steps slowed them)."""
     from oracle import oracle as O
     p = O.default_params(**WORKLOAD)
     size = state.phi.shape[0]
@@ -452,29 +453,30 @@ def main():
     n = args.size
     cells = n * n
     first = args.start + args.warmup
-    with kb.Solver(n, n, "f32", kb.default_params(**WORKLOAD)) as s:
-        # reach step S of the trajectory untimed (default kernel: bitwise the
-        # dense results), then W untimed and K timed dense steps
-        s.reset_seeds(**SEEDS)
-        s.step(args.start)
-        s.synchronize()
-        start_state = None
-        if args.cpu_steps > 0 and not args.headline_only:
-            from oracle import oracle as O
-            start_state = O.State(*s.fields())
-        s.set_kernel("fused_dense")
-        if args.warmup > 0:
-            s.step(args.warmup)
-        # the solver's stream is the only GPU work in this process: its
-        # synchronisation brackets the timed steps (torch is not loaded here;
-        # loading it beside the solver's runtime lengthened each launch by
-        # about 5 us)
-        s.synchronize()
-        t0 = time.perf_counter()
-        ev_ms = s.step_timed(args.steps)  # HIP events on the solver's stream
-        s.synchronize()
-        wall = time.perf_counter() - t0
-        st = s.stats()
+    # the headline solver stays open until the CPU sample is read from it at
+    # the end: a readback of its fields slowed the dense launches that
+    # followed by 3-15 % (3 x 64 MiB device-to-host, pageable or pinned
+    # alike; without one they hold 52.2 us per launch), so no leg runs
+    # after one
+    s = kb.Solver(n, n, "f32", kb.default_params(**WORKLOAD))
+    # reach step S of the trajectory untimed (default kernel: bitwise the
+    # dense results), then W untimed and K timed dense steps
+    s.reset_seeds(**SEEDS)
+    s.step(args.start)
+    s.synchronize()
+    s.set_kernel("fused_dense")
+    if args.warmup > 0:
+        s.step(args.warmup)
+    # the solver's stream is the only GPU work in this process: its
+    # synchronisation brackets the timed steps (torch is not loaded here;
+    # loading it beside the solver's runtime lengthened each launch by
+    # about 5 us)
+    s.synchronize()
+    t0 = time.perf_counter()
+    ev_ms = s.step_timed(args.steps)  # HIP events on the solver's stream
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    st = s.stats()
 
     per_launch = ev_ms / 1e3 / max(args.steps, 1)
     value = cells * args.steps / wall / 1e9
@@ -515,20 +517,26 @@ def main():
                         "sum_phi": st["sum_phi"], "nonfinite": st["nonfinite"]},
     }
     if args.headline_only:
+        s.close()
         print(json.dumps(rec), flush=True)
         return
     rec["ceilings"] = ceilings(kb, n)
     if args.horizon > 0:
         rec["trajectory"] = trajectory(kb, n, args.horizon)
-    if not args.no_fp64_check:
-        rec["fp64_rel_err_100_steps"] = fp64_error(kb, n)
     if args.batch_steps > 0:
         rec["batched"] = batched_vs_looped(kb, args.batch_steps, not args.no_looped)
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_vs_forward(kb)
     if args.fit_iterations > 0:
         rec["fit"] = fit_summary(kb, args.fit_iterations)
-    rec["cpu_baseline"] = (cpu_baseline(start_state, args.cpu_steps, args.start)
+    if not args.no_fp64_check:  # reads 4096^2 fields back: after the timed legs
+        rec["fp64_rel_err_100_steps"] = fp64_error(kb, n)
+    start_state = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        start_state = O.State(*s.fields())  # the state right after the timed steps
+    s.close()
+    rec["cpu_baseline"] = (cpu_baseline(start_state, args.cpu_steps, first + args.steps)
                            if start_state is not None else None)
     if args.cpu_steps > 0:
         rec["cpu_all_cores"] = cpu_all_cores(n, 6 * args.cpu_steps)
