@@ -141,6 +141,8 @@ SIGNATURES = {
     "kg_batch_front_profile": [H, I, ctypes.c_double, ctypes.c_double, I, ctypes.c_double, VP],
     "kg_batch_set_mode": [H, I, I],
     "kg_batch_get_mode": [H, PI, PI],
+    "kg_batch_set_chunk_shape": [H, I, I],
+    "kg_batch_get_chunk_shape": [H, PI, PI],
     "kg_batch_autotune": [H, I, PI, PI],
     "kg_batch_step": [H, I],
     "kg_batch_step_timed": [H, I, ctypes.POINTER(ctypes.c_double)],

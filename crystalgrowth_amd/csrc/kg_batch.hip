@@ -77,25 +77,25 @@ void free_batch(kg_batch_s *b)
 }
 
 // The tile table, in block order: block b steps the tile of entry b.  Every
-// member is cut into chunks of 2 x 2 tiles (chunk rows rotated by 3, as the
-// single-grid kernel orders its chunks; the shape comes from kg_step.hip,
-// whose batch liveness requests check the same chunks); the chunks, member
-// after member, are
-// dealt round-robin to the 8 XCDs (block b runs on XCD b % 8), each XCD
-// walks its chunks one after the other, and its sequence is padded to the
-// longest one with empty entries (member -1).  So every XCD steps tiles of
-// many members at any time, and tiles that share halos meet in one L2.
+// member is cut into chunks of cw x ch tiles (2 x 2 unless the batch chooses
+// another of kg_step.hip's batch shapes, whose liveness requests check the
+// same chunks; chunk rows rotated by 3, as the single-grid kernel orders its
+// chunks); the chunks, member after member, are dealt round-robin to the 8
+// XCDs (block b runs on XCD b % 8), each XCD walks its chunks one after the
+// other, and its sequence is padded to the longest one with empty entries
+// (member -1).  So every XCD steps tiles of many members at any time, and
+// tiles that share halos meet in one L2.
 constexpr int BNX = 8;
-const int BCW = kg::batch_chunk_w(), BCH = kg::batch_chunk_h(), BSKEW = kg::fused_chunk_skew();
+const int BSKEW = kg::fused_chunk_skew();
 
 // the chunks of every member in that order: (member, chunk column, chunk row)
-std::vector<int4> batch_chunks(const int *nx, const int *ny, int count)
+std::vector<int4> batch_chunks(const int *nx, const int *ny, int count, int cw, int ch)
 {
     const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
     std::vector<int4> out;
     for (int m = 0; m < count; ++m) {
         const int tx = (nx[m] + TX - 1) / TX, ty = (ny[m] + TY - 1) / TY;
-        const int ncx = (tx + BCW - 1) / BCW, ncy = (ty + BCH - 1) / BCH;
+        const int ncx = (tx + cw - 1) / cw, ncy = (ty + ch - 1) / ch;
         for (int cy = 0; cy < ncy; ++cy)
             for (int c = 0; c < ncx; ++c)
                 out.push_back(make_int4(m, (c + BSKEW * cy) % ncx, cy, 0));
@@ -103,20 +103,21 @@ std::vector<int4> batch_chunks(const int *nx, const int *ny, int count)
     return out;
 }
 
-// The tile table of chunks[order[0]], chunks[order[1]], ...: the j-th goes
-// to XCD j % 8.  *real = number of tiles (entries that are not padding).
+// The tile table of chunks[order[0]], chunks[order[1]], ... (cw x ch tiles
+// each): the j-th goes to XCD j % 8.  *real = number of tiles (entries that
+// are not padding).
 std::vector<int4> deal_chunks(const std::vector<int4> &chunks, const std::vector<int> &order,
-                              const int *nx, const int *ny, size_t *real)
+                              const int *nx, const int *ny, int cw, int ch, size_t *real)
 {
     const int TX = kg::fused_tile_x(), TY = kg::fused_tile_y();
     std::vector<std::vector<int4>> seq(BNX);
     size_t n = 0;
     for (size_t j = 0; j < order.size(); ++j) {
-        const int4 ch = chunks[(size_t)order[j]];
-        const int m = ch.x;
+        const int4 c = chunks[(size_t)order[j]];
+        const int m = c.x;
         const int tx = (nx[m] + TX - 1) / TX, ty = (ny[m] + TY - 1) / TY;
-        for (int k = 0; k < BCW * BCH; ++k) {
-            const int x = ch.y * BCW + k % BCW, y = ch.z * BCH + k / BCW;
+        for (int k = 0; k < cw * ch; ++k) {
+            const int x = c.y * cw + k % cw, y = c.z * ch + k / cw;
             if (x < tx && y < ty) {
                 seq[j % BNX].push_back(make_int4(m, x * TX, y * TY, 0));
                 ++n;
@@ -134,13 +135,112 @@ std::vector<int4> deal_chunks(const std::vector<int4> &chunks, const std::vector
     return table;
 }
 
-std::vector<int4> batch_tile_table(const int *nx, const int *ny, int count, size_t *real)
+std::vector<int4> batch_tile_table(const int *nx, const int *ny, int count, int cw, int ch,
+                                   size_t *real)
 {
-    const std::vector<int4> ch = batch_chunks(nx, ny, count);
-    std::vector<int> order(ch.size());
+    const std::vector<int4> chunks = batch_chunks(nx, ny, count, cw, ch);
+    std::vector<int> order(chunks.size());
     for (size_t j = 0; j < order.size(); ++j)
         order[j] = (int)j;
-    return deal_chunks(ch, order, nx, ny, real);
+    return deal_chunks(chunks, order, nx, ny, cw, ch, real);
+}
+
+// What depends on the chunk shape: the tile table, and the chunk lists'
+// chunks, flags, event and listed tables (batches of at least
+// SKIP_MIN_CHUNKS chunks).  Built beside the batch's own and exchanged with
+// them, so a failed build leaves the batch as it was.
+struct Tables {
+    int cw = 0, ch = 0;
+    int4 *tiles = nullptr;
+    int ntiles = 0;
+    std::vector<int4> chunk_list;
+    int4 *chunks_dev = nullptr;
+    int nchunks = 0;
+    uint8_t *live_flags = nullptr;
+    hipEvent_t live_ev = nullptr;
+    int4 *tiles_listed = nullptr, *tiles_stage = nullptr;
+    size_t listed_cap = 0;
+};
+
+void free_tables(Tables &t)
+{
+    for (void *p : {static_cast<void *>(t.tiles), static_cast<void *>(t.chunks_dev),
+                    static_cast<void *>(t.tiles_listed)})
+        if (p)
+            (void)hipFree(p);
+    if (t.tiles_stage)
+        (void)hipHostFree(t.tiles_stage);
+    if (t.live_flags)
+        (void)hipHostFree(t.live_flags);
+    if (t.live_ev)
+        (void)hipEventDestroy(t.live_ev);
+    t = Tables();
+}
+
+void swap_tables(kg_batch_s &b, Tables &t)
+{
+    std::swap(b.cw, t.cw);
+    std::swap(b.ch, t.ch);
+    std::swap(b.tiles, t.tiles);
+    std::swap(b.ntiles, t.ntiles);
+    std::swap(b.chunk_list, t.chunk_list);
+    std::swap(b.chunks_dev, t.chunks_dev);
+    std::swap(b.nchunks, t.nchunks);
+    std::swap(b.live_flags, t.live_flags);
+    std::swap(b.live_ev, t.live_ev);
+    std::swap(b.tiles_listed, t.tiles_listed);
+    std::swap(b.tiles_stage, t.tiles_stage);
+    std::swap(b.listed_cap, t.listed_cap);
+}
+
+// Builds the tables of chunk shape cw x ch into t (free_tables(t) after a
+// failure).  Uses the null stream: the batch's stream must be idle.
+int make_tables(const kg_batch_s &b, int cw, int ch, Tables &t)
+{
+    t.cw = cw;
+    t.ch = ch;
+    size_t real = 0;
+    const std::vector<int4> tiles =
+        batch_tile_table(b.nx.data(), b.ny.data(), b.count, cw, ch, &real);
+    if (tiles.size() > (size_t)(1 << 30))
+        return fail(KG_ERR_INVALID_ARGUMENT, "too many tiles");
+    t.ntiles = (int)tiles.size();
+    if (hipMalloc(&t.tiles, tiles.size() * sizeof(int4)) != hipSuccess) {
+        t.tiles = nullptr;
+        return fail(KG_ERR_OUT_OF_MEMORY, "hipMalloc of the batch tile table failed");
+    }
+    KG_HIP(hipMemcpy(t.tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+    // chunk lists (up to cw * ch tiles per chunk, every XCD padded to
+    // ceil(chunks / 8) chunks)
+    t.chunk_list = batch_chunks(b.nx.data(), b.ny.data(), b.count, cw, ch);
+    t.nchunks = (int)t.chunk_list.size();
+    if (t.nchunks < kg::SKIP_MIN_CHUNKS)
+        return KG_OK;
+    t.listed_cap = ((size_t)t.nchunks + BNX - 1) / BNX * BNX * cw * ch;
+    bool ok = hipMalloc(&t.chunks_dev, t.chunk_list.size() * sizeof(int4)) == hipSuccess;
+    if (!ok)
+        t.chunks_dev = nullptr;
+    ok = ok && hipMalloc(&t.tiles_listed, t.listed_cap * sizeof(int4)) == hipSuccess;
+    if (!ok)
+        t.tiles_listed = nullptr;
+    ok = ok && hipHostMalloc(reinterpret_cast<void **>(&t.tiles_stage),
+                             t.listed_cap * sizeof(int4), hipHostMallocDefault) == hipSuccess;
+    if (!ok)
+        t.tiles_stage = nullptr;
+    ok = ok && hipHostMalloc(reinterpret_cast<void **>(&t.live_flags), (size_t)t.nchunks,
+                             hipHostMallocCoherent) == hipSuccess;
+    if (!ok)
+        t.live_flags = nullptr;
+    ok = ok && hipEventCreateWithFlags(&t.live_ev, hipEventDisableTiming |
+                                                       hipEventDisableSystemFence) == hipSuccess;
+    if (!ok) {
+        t.live_ev = nullptr;
+        return fail(KG_ERR_OUT_OF_MEMORY, "allocation of the batch chunk lists failed");
+    }
+    std::memset(t.live_flags, 0, (size_t)t.nchunks);  // tag 0: never written
+    KG_HIP(hipMemcpy(t.chunks_dev, t.chunk_list.data(), t.chunk_list.size() * sizeof(int4),
+                     hipMemcpyHostToDevice));
+    return KG_OK;
 }
 
 template <typename R>
@@ -218,7 +318,7 @@ int batch_skip_cycle(kg_batch_s &b)
         } else {
             std::vector<int> order;
             order.reserve((size_t)b.nchunks);
-            for (int na = BCW * BCH; na >= 0; --na) {
+            for (int na = b.cw * b.ch; na >= 0; --na) {
                 const uint32_t want = (uint32_t)na << 1 | 1u;
                 for (int c = 0; c < b.nchunks; ++c)
                     if ((b.live_flags[c] & 31u) == want)
@@ -227,7 +327,7 @@ int batch_skip_cycle(kg_batch_s &b)
             if ((int)order.size() < b.nchunks) {
                 size_t real = 0;
                 const std::vector<int4> t =
-                    deal_chunks(b.chunk_list, order, b.nx.data(), b.ny.data(), &real);
+                    deal_chunks(b.chunk_list, order, b.nx.data(), b.ny.data(), b.cw, b.ch, &real);
                 if (t.size() <= b.listed_cap) {
                     std::copy(t.begin(), t.end(), b.tiles_stage);
                     if (!t.empty())
@@ -245,10 +345,10 @@ int batch_skip_cycle(kg_batch_s &b)
     hipError_t e = b.precision == KG_F64
                        ? kg::launch_batch_live<double>(
                              static_cast<const kg::StepArgs<double> *>(b.members), b.chunks_dev,
-                             b.nchunks, set, b.live_tag, b.live_flags, b.stream)
+                             b.nchunks, set, b.live_tag, b.live_flags, b.cw, b.ch, b.stream)
                        : kg::launch_batch_live<float>(
                              static_cast<const kg::StepArgs<float> *>(b.members), b.chunks_dev,
-                             b.nchunks, set, b.live_tag, b.live_flags, b.stream);
+                             b.nchunks, set, b.live_tag, b.live_flags, b.cw, b.ch, b.stream);
     if (e != hipSuccess)
         return hip_fail(e, "batch liveness launch");
     KG_HIP(hipEventRecord(b.live_ev, b.stream));
@@ -424,12 +524,13 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
     }
     b->total = total;
     size_t real_tiles = 0;
-    std::vector<int4> tiles = batch_tile_table(nx, ny, count, &real_tiles);
-    if (tiles.size() > (size_t)(1 << 30)) {
+    for (int m = 0; m < count; ++m)
+        real_tiles += (size_t)((nx[m] + kg::fused_tile_x() - 1) / kg::fused_tile_x()) *
+                      (size_t)((ny[m] + kg::fused_tile_y() - 1) / kg::fused_tile_y());
+    if (real_tiles > (size_t)(1 << 30)) {
         delete b;
         return fail(KG_ERR_INVALID_ARGUMENT, "too many tiles");
     }
-    b->ntiles = (int)tiles.size();
 
     hipError_t e = hipGetDevice(&b->device);
     if (e == hipSuccess)
@@ -455,7 +556,6 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
     b->stat_tiles = (int)real_tiles;
     if (hipMalloc(&b->members, args_bytes) != hipSuccess ||
         hipMalloc(&b->stat, 2 * real_tiles * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&b->tiles, tiles.size() * sizeof(int4)) != hipSuccess ||
         hipMalloc(&b->step_base, sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&b->partials, kg::reduce_partials_size() * sizeof(double)) != hipSuccess ||
         hipMalloc(&b->red, sizeof(kg::ReduceOut)) != hipSuccess) {
@@ -470,48 +570,19 @@ int kg_batch_create(int count, const int *nx, const int *ny, int precision,
         e = hipMemcpy(b->members, v.data(), args_bytes, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess)
-        e = hipMemcpy(b->tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
         e = hipMemset(b->step_base, 0, sizeof(uint32_t));
     if (e != hipSuccess) {
         free_batch(b);
         return hip_fail(e, "batch table upload");
     }
-    // chunk lists: state for the liveness requests and the listed tables
-    // (up to 8 tiles per chunk, every XCD padded to ceil(chunks / 8) chunks)
-    b->chunk_list = batch_chunks(nx, ny, count);
-    b->nchunks = (int)b->chunk_list.size();
-    if (b->nchunks >= kg::SKIP_MIN_CHUNKS) {
-        b->listed_cap = ((size_t)b->nchunks + BNX - 1) / BNX * BNX * BCW * BCH;
-        bool ok = hipMalloc(&b->chunks_dev, b->chunk_list.size() * sizeof(int4)) == hipSuccess;
-        if (!ok)
-            b->chunks_dev = nullptr;
-        ok = ok && hipMalloc(&b->tiles_listed, b->listed_cap * sizeof(int4)) == hipSuccess;
-        if (!ok)
-            b->tiles_listed = nullptr;
-        ok = ok && hipHostMalloc(reinterpret_cast<void **>(&b->tiles_stage),
-                                 b->listed_cap * sizeof(int4), hipHostMallocDefault) == hipSuccess;
-        if (!ok)
-            b->tiles_stage = nullptr;
-        ok = ok && hipHostMalloc(reinterpret_cast<void **>(&b->live_flags), (size_t)b->nchunks,
-                                 hipHostMallocCoherent) == hipSuccess;
-        if (!ok)
-            b->live_flags = nullptr;
-        ok = ok && hipEventCreateWithFlags(&b->live_ev, hipEventDisableTiming |
-                                                            hipEventDisableSystemFence) ==
-                       hipSuccess;
-        if (!ok) {
-            free_batch(b);
-            return fail(KG_ERR_OUT_OF_MEMORY, "allocation of the batch chunk lists failed");
-        }
-        std::memset(b->live_flags, 0, (size_t)b->nchunks);  // tag 0: never written
-        e = hipMemcpy(b->chunks_dev, b->chunk_list.data(), b->chunk_list.size() * sizeof(int4),
-                      hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            free_batch(b);
-            return hip_fail(e, "batch chunk upload");
-        }
+    // the tile table and the chunk lists' state, in the default chunk shape
+    Tables t;
+    if (int rc = make_tables(*b, kg::batch_chunk_w(), kg::batch_chunk_h(), t)) {
+        free_tables(t);
+        free_batch(b);
+        return rc;
     }
+    swap_tables(*b, t);
     for (int m = 0; m < count; ++m) {
         int rc = init_member(*b, m, 0, nx[m] / 2, ny[m] / 2, {}, 0.0);
         if (rc != KG_OK) {
@@ -771,6 +842,49 @@ int kg_batch_get_mode(kg_batch b, int *mode, int *graph_steps)
     return KG_OK;
 }
 
+int kg_batch_set_chunk_shape(kg_batch b, int chunk_w, int chunk_h)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    int shapes[16][2];
+    const int n = kg::batch_chunk_shapes(shapes, 16);
+    bool known = false;
+    for (int i = 0; i < n && i < 16; ++i)
+        known = known || (shapes[i][0] == chunk_w && shapes[i][1] == chunk_h);
+    if (!known) {
+        std::string list;
+        for (int i = 0; i < n && i < 16; ++i)
+            list += (i ? ", " : "") + std::to_string(shapes[i][0]) + "x" +
+                    std::to_string(shapes[i][1]);
+        return fail(KG_ERR_INVALID_ARGUMENT, "chunk shape must be one of " + list);
+    }
+    if (chunk_w == b->cw && chunk_h == b->ch)
+        return KG_OK;
+    KG_HIP(hipSetDevice(b->device));
+    KG_HIP(hipStreamSynchronize(b->stream));
+    Tables t;
+    if (int rc = make_tables(*b, chunk_w, chunk_h, t)) {
+        free_tables(t);
+        return rc;
+    }
+    swap_tables(*b, t);
+    free_tables(t);  // the old shape's
+    drop_graph(b);   // it launched the old tile table
+    drop_lists(*b);
+    return KG_OK;
+}
+
+int kg_batch_get_chunk_shape(kg_batch b, int *chunk_w, int *chunk_h)
+{
+    if (int rc = check_batch(b, 0, false))
+        return rc;
+    if (!chunk_w || !chunk_h)
+        return fail(KG_ERR_INVALID_ARGUMENT, "null output");
+    *chunk_w = b->cw;
+    *chunk_h = b->ch;
+    return KG_OK;
+}
+
 int kg_batch_step(kg_batch b, int nsteps)
 {
     if (int rc = check_batch(b, 0, false))
@@ -859,18 +973,25 @@ int kg_batch_autotune(kg_batch b, int trial_steps, int *mode, int *graph_steps)
     if (e == hipSuccess)
         e = hipMemcpyAsync(s + 2 * fb, b->theta, fb, hipMemcpyDeviceToDevice, b->stream);
     const uint64_t steps0 = b->steps;
-    const int mode0 = b->mode, g0 = b->graph_steps;
-    struct Cand { int mode, g; };
-    const Cand cands[] = {{KG_BATCH_LAUNCH, 0}, {KG_BATCH_GRAPH, 2}, {KG_BATCH_GRAPH, 8},
-                          {KG_BATCH_GRAPH, 32}};
+    // every launch form in every chunk shape
+    struct Cand { int mode, g, cw, ch; };
+    const Cand forms[] = {{KG_BATCH_LAUNCH, 0, 0, 0}, {KG_BATCH_GRAPH, 2, 0, 0},
+                          {KG_BATCH_GRAPH, 8, 0, 0}, {KG_BATCH_GRAPH, 32, 0, 0}};
+    int shapes[16][2];
+    const int nshapes = std::min(kg::batch_chunk_shapes(shapes, 16), 16);
+    std::vector<Cand> cands;
+    for (int k = 0; k < nshapes; ++k)
+        for (const Cand &f : forms)
+            if (f.mode == KG_BATCH_LAUNCH || f.g <= trial_steps)
+                cands.push_back({f.mode, f.g, shapes[k][0], shapes[k][1]});
     double best = 1e300;
-    Cand pick = {mode0, g0};
+    Cand pick = {b->mode, b->graph_steps, b->cw, b->ch};
     for (const Cand &cd : cands) {
         if (e != hipSuccess)
             break;
-        if (cd.mode == KG_BATCH_GRAPH && cd.g > trial_steps)
-            continue;
-        int rc = kg_batch_set_mode(b, cd.mode, cd.g);
+        int rc = kg_batch_set_chunk_shape(b, cd.cw, cd.ch);
+        if (rc == KG_OK)
+            rc = kg_batch_set_mode(b, cd.mode, cd.g);
         double ms = 0.0;
         if (rc == KG_OK)
             rc = kg_batch_step(b, cd.g > 0 ? cd.g : 2);  // warm up (capture, caches)
@@ -900,6 +1021,8 @@ int kg_batch_autotune(kg_batch b, int trial_steps, int *mode, int *graph_steps)
     (void)hipFree(save);
     if (e != hipSuccess)
         return hip_fail(e, "autotune state copy");
+    if (int rc = kg_batch_set_chunk_shape(b, pick.cw, pick.ch))
+        return rc;
     if (int rc = kg_batch_set_mode(b, pick.mode, pick.g))
         return rc;
     if (mode)

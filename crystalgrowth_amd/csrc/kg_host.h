@@ -110,6 +110,7 @@ struct kg_batch_s {
     // of the request in flight, and the tile table of the listed chunks on
     // the device with its pinned staging copy; ntiles_listed < 0: the full
     // table
+    int cw = 0, ch = 0;        // chunk shape (tiles) of the tile table and the lists
     std::vector<int4> chunk_list;
     int4 *chunks_dev = nullptr;
     int nchunks = 0;

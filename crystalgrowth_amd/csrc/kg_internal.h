@@ -109,9 +109,12 @@ bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
 // Liveness request over the chunks of a batch: flags[c] as a LiveReq writes
 // them, for chunk c = (member, chunk column, chunk row) of chunks, from the
 // status set stat_set (the input set of the next step).
+// Chunks are cw x ch tiles, a shape of batch_chunk_shapes (else
+// hipErrorInvalidValue).
 template <typename R>
 hipError_t launch_batch_live(const StepArgs<R> *members, const int4 *chunks, int nchunks,
-                             const uint32_t *stat_set, uint32_t tag, uint8_t *flags, hipStream_t s);
+                             const uint32_t *stat_set, uint32_t tag, uint8_t *flags, int cw,
+                             int ch, hipStream_t s);
 // Chunk grid of the fused step for an nx x ny grid (chunks_x * chunks_y chunks).
 int fused_chunks_x(int nx);
 int fused_chunks_y(int ny);
@@ -164,13 +167,16 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s);
 int fused_tile_x();
 int fused_tile_y();
 // Chunk shape of the fused step's chunk lists (tiles) and the rotation of
-// successive chunk rows; a batch cuts its members into chunks of
-// batch_chunk_w x batch_chunk_h tiles, which its liveness requests check.
+// successive chunk rows; a batch cuts its members into chunks of one of the
+// batch_chunk_shapes (writes up to cap (width, height) pairs, returns how
+// many there are), batch_chunk_w x batch_chunk_h unless it chooses another,
+// and its liveness requests check the same chunks.
 int fused_chunk_w();
 int fused_chunk_h();
 int fused_chunk_skew();
 int batch_chunk_w();
 int batch_chunk_h();
+int batch_chunk_shapes(int (*out)[2], int cap);
 
 // Batched step over a tile table (member, x0, y0); the step counter is
 // *step_base + step_add and its parity picks the ping-pong direction.

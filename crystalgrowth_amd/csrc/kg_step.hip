@@ -1041,19 +1041,15 @@ constexpr int LIVE_NEED = 2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2;
 static_assert(SKIP_MARGIN_X == 1 && SKIP_MARGIN_Y == 1 && LIVE_NEED <= TX && LIVE_NEED <= TY,
               "one full margin tile spans a margin; the window logic assumes one");
 static_assert(CHUNK_W * CHUNK_H < 16, "active-tile count fits 4 bits");
-// A batch deals its own chunks, smaller than the single grid's: its members
-// are small (a 128^2 member has 6 tiles) and mostly busy, so finer chunks
-// leave fewer idle tiles in a listed launch.  Measured on the 512-member
-// ensemble over 5000 steps: 2 x 2 300 Gcell-updates/s, 2 x 4 293, 4 x 2
-// 288, 4 x 1 292, 8 x 1 280.
-#ifndef KG_BATCH_CHUNK_W
-#define KG_BATCH_CHUNK_W 2
-#endif
-#ifndef KG_BATCH_CHUNK_H
-#define KG_BATCH_CHUNK_H 2
-#endif
-constexpr int BATCH_CHUNK_W = KG_BATCH_CHUNK_W, BATCH_CHUNK_H = KG_BATCH_CHUNK_H;
-static_assert(BATCH_CHUNK_W * BATCH_CHUNK_H < 16, "active-tile count fits 4 bits");
+// A batch deals its own chunks, by default smaller than the single grid's:
+// its members are small (a 128^2 member has 6 tiles) and mostly busy, so
+// finer chunks leave fewer idle tiles in a listed launch.  Measured on the
+// 512-member ensemble over 5000 steps: 2 x 2 300 Gcell-updates/s, 2 x 4 293,
+// 4 x 2 288, 4 x 1 292, 8 x 1 280.  The shape is the batch's choice (set
+// directly or by its autotuner) among these; the liveness request is
+// compiled for each.
+constexpr int BATCH_CHUNK_W = 2, BATCH_CHUNK_H = 2;
+constexpr int BATCH_SHAPES[][2] = {{2, 2}, {2, 4}, {4, 2}, {4, 1}, {8, 1}};
 __device__ __forceinline__ int live_wrap(int v, int n, int boundary)
 {
     if (boundary == 1)
@@ -1117,7 +1113,7 @@ __device__ __forceinline__ void live_request(const StepArgs<R> &a, const LiveReq
 // The same request over the chunks of a batch (chunks[c] = member, chunk
 // column, chunk row): the member's input status set is stat_set + its
 // tile_base.  A member without status has every chunk live and heavy.
-template <typename R>
+template <typename R, int CW, int CH>
 __global__ __launch_bounds__(256) void k_batch_live(const StepArgs<R> *__restrict__ members,
                                                     const int4 *__restrict__ chunks, int nchunks,
                                                     const uint32_t *__restrict__ stat_set,
@@ -1128,10 +1124,11 @@ __global__ __launch_bounds__(256) void k_batch_live(const StepArgs<R> *__restric
         return;
     const int4 d = chunks[c];
     const StepArgs<R> &a = members[d.x];
-    uint8_t f = (uint8_t)(tag << 5 | (uint32_t)(BATCH_CHUNK_W * BATCH_CHUNK_H) << 1 | 1u);
+    static_assert(CW * CH < 16, "active-tile count fits 4 bits");
+    uint8_t f = (uint8_t)(tag << 5 | (uint32_t)(CW * CH) << 1 | 1u);
     if (a.stat_in != nullptr)
-        f = chunk_flag<BATCH_CHUNK_W, BATCH_CHUNK_H>(stat_set + a.tile_base, a.tiles_x, a.tiles_y,
-                                                     a.nx, a.ny, a.boundary, d.y, d.z, tag);
+        f = chunk_flag<CW, CH>(stat_set + a.tile_base, a.tiles_x, a.tiles_y, a.nx, a.ny,
+                               a.boundary, d.y, d.z, tag);
     __hip_atomic_store(flags + c, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
@@ -1585,15 +1582,34 @@ bool live_chunk_order(int nx, int ny, const uint8_t *flags, uint32_t tag,
     return true;
 }
 
+template <typename R, int CW, int CH>
+hipError_t launch_batch_live_shape(const StepArgs<R> *members, const int4 *chunks, int nchunks,
+                                   const uint32_t *stat_set, uint32_t tag, uint8_t *flags,
+                                   hipStream_t s)
+{
+    hipLaunchKernelGGL((k_batch_live<R, CW, CH>), dim3((nchunks + 255) / 256), dim3(256), 0, s,
+                       members, chunks, nchunks, stat_set, tag, flags);
+    return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_batch_live(const StepArgs<R> *members, const int4 *chunks, int nchunks,
-                             const uint32_t *stat_set, uint32_t tag, uint8_t *flags, hipStream_t s)
+                             const uint32_t *stat_set, uint32_t tag, uint8_t *flags, int cw,
+                             int ch, hipStream_t s)
 {
     if (nchunks <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL((k_batch_live<R>), dim3((nchunks + 255) / 256), dim3(256), 0, s, members,
-                       chunks, nchunks, stat_set, tag, flags);
-    return hipGetLastError();
+#define KG_BATCH_SHAPE(W, H)                                                                  \
+    if (cw == W && ch == H)                                                                   \
+        return launch_batch_live_shape<R, W, H>(members, chunks, nchunks, stat_set, tag, flags, \
+                                                s);
+    KG_BATCH_SHAPE(2, 2)
+    KG_BATCH_SHAPE(2, 4)
+    KG_BATCH_SHAPE(4, 2)
+    KG_BATCH_SHAPE(4, 1)
+    KG_BATCH_SHAPE(8, 1)
+#undef KG_BATCH_SHAPE
+    return hipErrorInvalidValue;  // not a shape of BATCH_SHAPES
 }
 
 int fused_chunks_x(int nx) { return ((nx + TX - 1) / TX + CHUNK_W - 1) / CHUNK_W; }
@@ -1605,6 +1621,15 @@ int fused_chunk_w() { return CHUNK_W; }
 int fused_chunk_h() { return CHUNK_H; }
 int batch_chunk_w() { return BATCH_CHUNK_W; }
 int batch_chunk_h() { return BATCH_CHUNK_H; }
+int batch_chunk_shapes(int (*out)[2], int cap)
+{
+    const int n = (int)(sizeof(BATCH_SHAPES) / sizeof(BATCH_SHAPES[0]));
+    for (int i = 0; i < n && i < cap; ++i) {
+        out[i][0] = BATCH_SHAPES[i][0];
+        out[i][1] = BATCH_SHAPES[i][1];
+    }
+    return n;
+}
 int fused_chunk_skew() { return CHUNK_SKEW; }
 
 bool fused_status_ok(int nx, int ny, int boundary)
@@ -1767,7 +1792,8 @@ hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double 
     template hipError_t launch_step_fused<R>(const StepArgs<R> &, hipStream_t, const LiveList *, \
                                              const LiveReq *);                                \
     template hipError_t launch_batch_live<R>(const StepArgs<R> *, const int4 *, int,             \
-                                             const uint32_t *, uint32_t, uint8_t *, hipStream_t); \
+                                             const uint32_t *, uint32_t, uint8_t *, int, int,     \
+                                             hipStream_t);                                        \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
                                              const uint32_t *, uint32_t, hipStream_t);           \
     template hipError_t launch_step_batch_direct<R>(const StepArgs<R> *, const int4 *, int,      \

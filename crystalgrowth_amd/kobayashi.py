@@ -386,7 +386,23 @@ class Batch:
         check(lib().kg_batch_get_mode(self._h, ctypes.byref(m), ctypes.byref(g)))
         return ("launch" if m.value == KG_BATCH_LAUNCH else "graph"), g.value
 
+    CHUNK_SHAPES = ((2, 2), (2, 4), (4, 2), (4, 1), (8, 1))
+
+    def set_chunk_shape(self, width: int, height: int) -> None:
+        """Cut every member into chunks of width x height tiles (one of
+        CHUNK_SHAPES; kg_batch_set_chunk_shape)."""
+        check(lib().kg_batch_set_chunk_shape(self._h, int(width), int(height)))
+
+    @property
+    def chunk_shape(self):
+        w, h = ctypes.c_int(), ctypes.c_int()
+        check(lib().kg_batch_get_chunk_shape(self._h, ctypes.byref(w), ctypes.byref(h)))
+        return w.value, h.value
+
     def autotune(self, trial_steps: int = 16):
+        """Times every launch form in every chunk shape from the current
+        state, keeps the fastest and restores the state; returns the launch
+        form (chunk_shape gives the shape)."""
         m, g = ctypes.c_int(), ctypes.c_int()
         check(lib().kg_batch_autotune(self._h, int(trial_steps), ctypes.byref(m), ctypes.byref(g)))
         return ("launch" if m.value == KG_BATCH_LAUNCH else "graph"), g.value

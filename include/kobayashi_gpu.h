@@ -248,8 +248,18 @@ int kg_batch_front_profile(kg_batch b, int member, double cx, double cy, int n, 
 /* mode KG_BATCH_LAUNCH or KG_BATCH_GRAPH (graph_steps even, >= 2) */
 int kg_batch_set_mode(kg_batch b, int mode, int graph_steps);
 int kg_batch_get_mode(kg_batch b, int *mode, int *graph_steps);
-/* Times trial_steps steps of each launch configuration on a copy of the
- * current state, keeps the fastest (the state is restored afterwards). */
+/* Chunk shape (in 64 x 30 tiles) into which every member is cut: the step
+ * launch deals the chunks round-robin to the XCDs, and in KG_BATCH_LAUNCH
+ * mode the chunk lists skip whole chunks.  One of 2x2 (the default), 2x4,
+ * 4x2, 4x1, 8x1; results are bitwise the same in every shape. */
+int kg_batch_set_chunk_shape(kg_batch b, int chunk_w, int chunk_h);
+int kg_batch_get_chunk_shape(kg_batch b, int *chunk_w, int *chunk_h);
+/* Times trial_steps steps of each launch configuration (launch form and
+ * chunk shape) on a copy of the current state and keeps the fastest (the
+ * state is restored afterwards; kg_batch_get_chunk_shape reports the shape
+ * it picked).  The chunk lists take effect within two liveness periods
+ * (14 steps) of the trial's start, so trials shorter than that time the
+ * launch of every chunk. */
 int kg_batch_autotune(kg_batch b, int trial_steps, int *mode, int *graph_steps);
 int kg_batch_step(kg_batch b, int nsteps);
 int kg_batch_step_timed(kg_batch b, int nsteps, double *ms);

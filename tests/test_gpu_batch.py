@@ -115,6 +115,7 @@ def test_autotune_restores_state(kb):
         before = [b.fields(m) for m in range(len(sizes))]
         mode, g = b.autotune(8)
         assert mode in ("launch", "graph") and (mode == "launch" or g >= 2)
+        assert b.chunk_shape in kb.Batch.CHUNK_SHAPES
         assert b.step_count == 3
         for m in range(len(sizes)):
             for x, y in zip(b.fields(m), before[m]):
@@ -163,13 +164,15 @@ def test_member_with_wide_stored_angles(kb):
         assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_chunk_lists_match_plain_solver_bitwise(kb, prec):
+@pytest.mark.parametrize("prec,shape", [("f32", (2, 2)), ("f64", (2, 2)), ("f32", (2, 4)),
+                                        ("f32", (4, 2)), ("f32", (4, 1)), ("f32", (8, 1))])
+def test_chunk_lists_match_plain_solver_bitwise(kb, prec, shape):
     """Launch mode over a batch of 256+ chunks takes chunk lists (members'
     far fields left out, heaviest chunks first): every member still evolves
-    bit for bit like a dense plain handle.  Members differ in size, boundary
-    and partial edge tiles; nuclei sit off centre so fronts cross members'
-    chunk edges and, on periodic members, wrap."""
+    bit for bit like a dense plain handle, in every chunk shape (the fewest
+    chunks, 262 at 2 x 4, still take lists).  Members differ in size,
+    boundary and partial edge tiles; nuclei sit off centre so fronts cross
+    members' chunk edges and, on periodic members, wrap."""
     sizes = [(1024, 960), (770, 610), (1030, 901), (640, 480), (900, 905)]
     params = [
         kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1, seed=1, instance=0),
@@ -181,6 +184,9 @@ def test_chunk_lists_match_plain_solver_bitwise(kb, prec):
     spots = [(100, 120), (766, 3), (5, 895), (320, 240), (899, 450)]
     marks = [1, 9, 40, 151, 400]
     with kb.Batch(sizes, params, prec) as b:
+        assert b.chunk_shape == (2, 2)
+        b.set_chunk_shape(*shape)
+        assert b.chunk_shape == shape
         b.set_mode("launch")
         for m, (cx, cy) in enumerate(spots):
             b.reset_disk(m, cx, cy, 6.0)
@@ -201,3 +207,28 @@ def test_chunk_lists_match_plain_solver_bitwise(kb, prec):
                 for name, x, y in zip(("phi", "T", "theta"), got[mk][m], s.fields()):
                     assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (m, mk, name)
     assert stats["listed_launches"] >= 350 and stats["stale_answers"] == 0, stats
+
+
+def test_chunk_shape_changes_mid_run(kb):
+    """The chunk shape changed between steps, in both launch forms, leaves
+    every member bit for bit on a plain handle's trajectory; an unknown
+    shape is refused and changes nothing."""
+    sizes, params = members(kb)
+    init = initial(sizes, "f32")
+    with kb.Batch(sizes, params, "f32") as b:
+        for m, st in enumerate(init):
+            b.set_fields(m, *st)
+        with pytest.raises(kb.KGError):
+            b.set_chunk_shape(3, 3)
+        assert b.chunk_shape == (2, 2)
+        done = 0
+        for shape, mode, g, n in [((8, 1), "launch", 0, 5), ((4, 2), "graph", 2, 4),
+                                  ((2, 4), "graph", 2, 6), ((2, 2), "launch", 0, 3)]:
+            b.set_chunk_shape(*shape)
+            b.set_mode(mode, g)
+            b.step(n)
+            done += n
+        for m, (nx, ny) in enumerate(sizes):
+            ref = plain(kb, nx, ny, "f32", params[m], init[m], done)
+            for x, y in zip(b.fields(m), ref):
+                assert np.array_equal(x, y), m
