@@ -79,7 +79,7 @@ def parse():
     ap.add_argument("--start", type=int, default=10000,
                     help="trajectory step the dense window starts from (S; reached untimed)")
     ap.add_argument("--size", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=10,
+    ap.add_argument("--cpu-steps", type=int, default=20,
                     help="steps of the CPU oracle sample (0 skips the CPU baseline)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline window (for profiler runs)")
