@@ -22,7 +22,7 @@ KG_ERR_SIZE = -5
 KG_F32, KG_F64 = 0, 1
 KG_PERIODIC, KG_ZERO_FLUX = 0, 1
 KG_KERNEL_FUSED, KG_KERNEL_TWO_PASS, KG_KERNEL_FUSED_DENSE = 0, 1, 2
-KG_BATCH_LAUNCH, KG_BATCH_GRAPH = 0, 1
+KG_BATCH_LAUNCH, KG_BATCH_GRAPH, KG_BATCH_GRAPH_LISTS = 0, 1, 2
 
 
 class KGError(RuntimeError):

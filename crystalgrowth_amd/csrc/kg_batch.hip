@@ -57,7 +57,7 @@ void free_batch(kg_batch_s *b)
     b->adj = nullptr;
     void *bufs[] = {b->phi[0], b->phi[1], b->T[0], b->T[1], b->theta, b->members,
                     b->tiles, b->step_base, b->partials, b->red, b->disks, b->stat,
-                    b->chunks_dev, b->tiles_listed};
+                    b->chunks_dev, b->tiles_listed, b->dflags2, b->dcount, b->dperiods};
     for (void *p : bufs)
         if (p)
             (void)hipFree(p);
@@ -160,12 +160,19 @@ struct Tables {
     hipEvent_t live_ev = nullptr;
     int4 *tiles_listed = nullptr, *tiles_stage = nullptr;
     size_t listed_cap = 0;
+    uint8_t *dflags2 = nullptr;
+    int *dcount = nullptr;
+    uint32_t *dperiods = nullptr;
 };
+
+// bytes of the two device flag buffers, rounded up to whole words
+size_t dflags_bytes(int nchunks) { return (2 * (size_t)nchunks + 3) / 4 * 4; }
 
 void free_tables(Tables &t)
 {
     for (void *p : {static_cast<void *>(t.tiles), static_cast<void *>(t.chunks_dev),
-                    static_cast<void *>(t.tiles_listed)})
+                    static_cast<void *>(t.tiles_listed), static_cast<void *>(t.dflags2),
+                    static_cast<void *>(t.dcount), static_cast<void *>(t.dperiods)})
         if (p)
             (void)hipFree(p);
     if (t.tiles_stage)
@@ -191,6 +198,9 @@ void swap_tables(kg_batch_s &b, Tables &t)
     std::swap(b.tiles_listed, t.tiles_listed);
     std::swap(b.tiles_stage, t.tiles_stage);
     std::swap(b.listed_cap, t.listed_cap);
+    std::swap(b.dflags2, t.dflags2);
+    std::swap(b.dcount, t.dcount);
+    std::swap(b.dperiods, t.dperiods);
 }
 
 // Builds the tables of chunk shape cw x ch into t (free_tables(t) after a
@@ -233,13 +243,48 @@ int make_tables(const kg_batch_s &b, int cw, int ch, Tables &t)
         t.live_flags = nullptr;
     ok = ok && hipEventCreateWithFlags(&t.live_ev, hipEventDisableTiming |
                                                        hipEventDisableSystemFence) == hipSuccess;
-    if (!ok) {
+    if (!ok)
         t.live_ev = nullptr;
+    ok = ok && hipMalloc(&t.dflags2, dflags_bytes(t.nchunks)) == hipSuccess;
+    if (!ok)
+        t.dflags2 = nullptr;
+    ok = ok && hipMalloc(&t.dcount, sizeof(int)) == hipSuccess;
+    if (!ok)
+        t.dcount = nullptr;
+    ok = ok && hipMalloc(&t.dperiods, sizeof(uint32_t)) == hipSuccess;
+    if (!ok) {
+        t.dperiods = nullptr;
         return fail(KG_ERR_OUT_OF_MEMORY, "allocation of the batch chunk lists failed");
     }
     std::memset(t.live_flags, 0, (size_t)t.nchunks);  // tag 0: never written
     KG_HIP(hipMemcpy(t.chunks_dev, t.chunk_list.data(), t.chunk_list.size() * sizeof(int4),
                      hipMemcpyHostToDevice));
+    KG_HIP(hipMemset(t.dflags2, 0, dflags_bytes(t.nchunks)));  // no answer
+    KG_HIP(hipMemset(t.dcount, 0xff, sizeof(int)));            // -1: the full table
+    KG_HIP(hipMemset(t.dperiods, 0, sizeof(uint32_t)));
+    return KG_OK;
+}
+
+// The device lists' arguments (graph mode), or null without them.
+bool dev_lists(const kg_batch_s &b, kg::DeviceLists &dl)
+{
+    if (b.dflags2 == nullptr)
+        return false;
+    dl = kg::DeviceLists{b.chunks_dev, b.nchunks, b.cw, b.ch, b.dflags2, b.tiles_listed,
+                         (int)b.listed_cap, b.dcount, b.dperiods, b.stat, b.stat_tiles};
+    return true;
+}
+
+// Answers and lists kept on the device are void: flags to tag 0 (never a
+// request's), length to -1 (the full table).  Fill kernels, queued on the
+// batch's stream.
+int clear_dev_lists(kg_batch_s &b)
+{
+    if (b.dflags2 == nullptr)
+        return KG_OK;
+    KG_HIP(kg::launch_fill_u32(reinterpret_cast<uint32_t *>(b.dflags2),
+                               dflags_bytes(b.nchunks) / 4, 0u, b.stream));
+    KG_HIP(kg::launch_fill_u32(reinterpret_cast<uint32_t *>(b.dcount), 1, 0xffffffffu, b.stream));
     return KG_OK;
 }
 
@@ -293,7 +338,7 @@ int ensure_status(kg_batch_s &b)
     KG_HIP(kg::launch_fill_u32(b.stat, 2 * (size_t)b.stat_tiles, 0u, b.stream));
     b.stat_valid = true;
     drop_lists(b);
-    return KG_OK;
+    return clear_dev_lists(b);
 }
 
 // The chunk lists of the plain solver (kg_api.hip, skip_cycle) over the
@@ -364,6 +409,10 @@ int launch_steps(kg_batch_s &b, int n)
     const bool lists = b.mode == KG_BATCH_LAUNCH && b.live_flags != nullptr;
     if (!lists)
         drop_lists(b);
+    // graph modes: the steps between whole replays run the graph's own
+    // kernels, so the device lists see every step
+    kg::DeviceLists dl;
+    const bool dev = b.mode == KG_BATCH_GRAPH_LISTS && dev_lists(b, dl);
     for (int s = 0; s < n; ++s) {
         if (lists && b.steps % kg::SKIP_PERIOD == 0)
             if (int rc = batch_skip_cycle(b))
@@ -373,14 +422,14 @@ int launch_steps(kg_batch_s &b, int n)
         const int nt = listed ? b.ntiles_listed : b.ntiles;
         const uint32_t add = (uint32_t)(b.steps - b.base_host);
         hipError_t e = hipSuccess;
-        if (nt > 0)
+        if (nt > 0 || dev)
             e = b.precision == KG_F64
                     ? kg::launch_step_batch<double>(
                           static_cast<const kg::StepArgs<double> *>(b.members), tiles, nt,
-                          b.step_base, add, b.stream)
+                          b.step_base, add, b.stream, dev ? &dl : nullptr)
                     : kg::launch_step_batch<float>(
                           static_cast<const kg::StepArgs<float> *>(b.members), tiles, nt,
-                          b.step_base, add, b.stream);
+                          b.step_base, add, b.stream, dev ? &dl : nullptr);
         if (e != hipSuccess)
             return hip_fail(e, "batch step launch");
         if (listed)
@@ -393,14 +442,19 @@ int launch_steps(kg_batch_s &b, int n)
 int capture_graph(kg_batch_s &b)
 {
     drop_graph(&b);
+    kg::DeviceLists dl;
+    const kg::DeviceLists *pdl =
+        b.mode == KG_BATCH_GRAPH_LISTS && dev_lists(b, dl) ? &dl : nullptr;
     KG_HIP(hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int s = 0; s < b.graph_steps && e == hipSuccess; ++s)
         e = b.precision == KG_F64
                 ? kg::launch_step_batch<double>(static_cast<const kg::StepArgs<double> *>(b.members),
-                                                b.tiles, b.ntiles, b.step_base, (uint32_t)s, b.stream)
+                                                b.tiles, b.ntiles, b.step_base, (uint32_t)s,
+                                                b.stream, pdl)
                 : kg::launch_step_batch<float>(static_cast<const kg::StepArgs<float> *>(b.members),
-                                               b.tiles, b.ntiles, b.step_base, (uint32_t)s, b.stream);
+                                               b.tiles, b.ntiles, b.step_base, (uint32_t)s,
+                                               b.stream, pdl);
     if (e == hipSuccess)
         e = kg::launch_advance(b.step_base, (uint32_t)b.graph_steps, b.stream);
     hipGraph_t g = nullptr;
@@ -417,7 +471,7 @@ int capture_graph(kg_batch_s &b)
 
 int step_impl(kg_batch_s &b, int nsteps)
 {
-    if (b.mode == KG_BATCH_GRAPH && nsteps >= b.graph_steps) {
+    if (b.mode != KG_BATCH_LAUNCH && nsteps >= b.graph_steps) {
         if (!b.exec)
             if (int rc = capture_graph(b))
                 return rc;
@@ -430,7 +484,7 @@ int step_impl(kg_batch_s &b, int nsteps)
         }
         if (int rc = ensure_status(b))
             return rc;
-        drop_lists(b);  // the graph steps every tile
+        drop_lists(b);  // the host's lists: launch mode only
         while (nsteps >= b.graph_steps) {
             KG_HIP(hipGraphLaunch(b.exec, b.stream));
             b.steps += (uint64_t)b.graph_steps;
@@ -818,16 +872,20 @@ int kg_batch_set_mode(kg_batch b, int mode, int graph_steps)
 {
     if (int rc = check_batch(b, 0, false))
         return rc;
-    if (mode != KG_BATCH_LAUNCH && mode != KG_BATCH_GRAPH)
-        return fail(KG_ERR_INVALID_ARGUMENT, "mode must be KG_BATCH_LAUNCH or KG_BATCH_GRAPH");
-    if (mode == KG_BATCH_GRAPH && (graph_steps < 2 || (graph_steps & 1) || graph_steps > 4096))
+    if (mode != KG_BATCH_LAUNCH && mode != KG_BATCH_GRAPH && mode != KG_BATCH_GRAPH_LISTS)
+        return fail(KG_ERR_INVALID_ARGUMENT,
+                    "mode must be KG_BATCH_LAUNCH, KG_BATCH_GRAPH or KG_BATCH_GRAPH_LISTS");
+    if (mode != KG_BATCH_LAUNCH && (graph_steps < 2 || (graph_steps & 1) || graph_steps > 4096))
         return fail(KG_ERR_INVALID_ARGUMENT, "graph_steps must be even and in [2, 4096]");
     KG_HIP(hipSetDevice(b->device));
     KG_HIP(hipStreamSynchronize(b->stream));
     if (mode != b->mode || graph_steps != b->graph_steps)
         drop_graph(b);
+    if (mode != b->mode)  // steps of the other mode did not keep the device lists
+        if (int rc = clear_dev_lists(*b))
+            return rc;
     b->mode = mode;
-    b->graph_steps = mode == KG_BATCH_GRAPH ? graph_steps : 0;
+    b->graph_steps = mode != KG_BATCH_LAUNCH ? graph_steps : 0;
     return KG_OK;
 }
 
@@ -904,7 +962,7 @@ int kg_batch_step_timed(kg_batch b, int nsteps, double *ms)
     KG_HIP(hipSetDevice(b->device));
     // a graph replay may first upload the counter synchronously; do it
     // outside the timed region
-    if (b->mode == KG_BATCH_GRAPH && nsteps >= b->graph_steps && b->base_host != b->steps) {
+    if (b->mode != KG_BATCH_LAUNCH && nsteps >= b->graph_steps && b->base_host != b->steps) {
         b->base_tmp = (uint32_t)b->steps;
         KG_HIP(hipMemcpyAsync(b->step_base, &b->base_tmp, sizeof(uint32_t),
                               hipMemcpyHostToDevice, b->stream));
@@ -937,7 +995,14 @@ int kg_batch_get_skip_stats(kg_batch b, uint64_t *listed_launches, uint64_t *sta
         return rc;
     if (!listed_launches || !stale_answers)
         return fail(KG_ERR_INVALID_ARGUMENT, "null output");
-    *listed_launches = b->listed_launches;
+    uint32_t periods = 0;
+    if (b->dperiods) {
+        KG_HIP(hipSetDevice(b->device));
+        KG_HIP(hipStreamSynchronize(b->stream));
+        KG_HIP(hipMemcpy(&periods, b->dperiods, sizeof(periods), hipMemcpyDeviceToHost));
+    }
+    // a list made on the device serves the SKIP_PERIOD launches after it
+    *listed_launches = b->listed_launches + (uint64_t)periods * kg::SKIP_PERIOD;
     *stale_answers = b->live_stale;
     return KG_OK;
 }
@@ -975,8 +1040,9 @@ int kg_batch_autotune(kg_batch b, int trial_steps, int *mode, int *graph_steps)
     const uint64_t steps0 = b->steps;
     // every launch form in every chunk shape
     struct Cand { int mode, g, cw, ch; };
-    const Cand forms[] = {{KG_BATCH_LAUNCH, 0, 0, 0}, {KG_BATCH_GRAPH, 2, 0, 0},
-                          {KG_BATCH_GRAPH, 8, 0, 0}, {KG_BATCH_GRAPH, 32, 0, 0}};
+    const Cand forms[] = {{KG_BATCH_LAUNCH, 0, 0, 0},      {KG_BATCH_GRAPH, 2, 0, 0},
+                          {KG_BATCH_GRAPH, 8, 0, 0},       {KG_BATCH_GRAPH, 32, 0, 0},
+                          {KG_BATCH_GRAPH_LISTS, 8, 0, 0}, {KG_BATCH_GRAPH_LISTS, 32, 0, 0}};
     int shapes[16][2];
     const int nshapes = std::min(kg::batch_chunk_shapes(shapes, 16), 16);
     std::vector<Cand> cands;

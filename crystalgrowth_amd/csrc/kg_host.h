@@ -121,6 +121,11 @@ struct kg_batch_s {
     int4 *tiles_listed = nullptr, *tiles_stage = nullptr;
     size_t listed_cap = 0;
     int ntiles_listed = -1;
+    // the same lists kept on the device in KG_BATCH_GRAPH mode (kg_step.hip,
+    // k_batch_compact): two flag buffers, the listed length, lists made
+    uint8_t *dflags2 = nullptr;
+    int *dcount = nullptr;
+    uint32_t *dperiods = nullptr;
     uint64_t listed_launches = 0, live_stale = 0;
 
     size_t esize() const { return precision == KG_F64 ? sizeof(double) : sizeof(float); }

@@ -178,11 +178,31 @@ int batch_chunk_w();
 int batch_chunk_h();
 int batch_chunk_shapes(int (*out)[2], int cap);
 
+// A batch's chunk lists kept on the device: its chunks (member, chunk
+// column, chunk row) of cw x ch tiles, two flag buffers of nchunks bytes,
+// the listed table (listed_cap entries), its length (-1: the full table),
+// a count of the lists made, and the tile status sets (stat_tiles words
+// each).
+struct DeviceLists {
+    const int4 *chunks;
+    int nchunks, cw, ch;
+    uint8_t *flags2;
+    int4 *listed;
+    int listed_cap;
+    int *count;
+    uint32_t *periods;
+    const uint32_t *stat;
+    int stat_tiles;
+};
 // Batched step over a tile table (member, x0, y0); the step counter is
-// *step_base + step_add and its parity picks the ping-pong direction.
+// *step_base + step_add and its parity picks the ping-pong direction.  With
+// dl, the launch is preceded by the device lists' list and request kernels
+// (both return at once except at multiples of SKIP_PERIOD) and uses the
+// listed table when there is one.
 template <typename R>
 hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
-                             const uint32_t *step_base, uint32_t step_add, hipStream_t s);
+                             const uint32_t *step_base, uint32_t step_add, hipStream_t s,
+                             const DeviceLists *dl = nullptr);
 // Batched step with the members' pointers as given and an absolute step.
 template <typename R>
 hipError_t launch_step_batch_direct(const StepArgs<R> *members, const int4 *tiles, int ntiles,

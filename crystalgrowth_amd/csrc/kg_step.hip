@@ -1132,6 +1132,157 @@ __global__ __launch_bounds__(256) void k_batch_live(const StepArgs<R> *__restric
     __hip_atomic_store(flags + c, f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Chunk lists of a batch kept on the device (KG_BATCH_GRAPH: no host wait,
+// so a captured graph keeps skipping).  The same request and answer cycle as
+// the host's (kg_batch.hip, batch_skip_cycle), driven by the device step
+// counter t = *step_base + step_add: at every t = q * SKIP_PERIOD,
+// k_batch_compact turns the answer of request q - 1 into the listed table
+// for steps t .. t + SKIP_PERIOD - 1, and k_batch_live_dev asks request q.
+// Request q writes flag buffer q & 1 with tag q % LIVE_TAGS + 1; an answer
+// whose flags do not all carry the expected tag (cleared buffers: tag 0) is
+// no answer, and the full table is used.  At other steps both return at once.
+template <typename R, int CW, int CH>
+__global__ __launch_bounds__(256) void k_batch_live_dev(const StepArgs<R> *__restrict__ members,
+                                                        const int4 *__restrict__ chunks,
+                                                        int nchunks,
+                                                        const uint32_t *__restrict__ stat,
+                                                        int stat_tiles,
+                                                        const uint32_t *__restrict__ step_base,
+                                                        uint32_t step_add,
+                                                        uint8_t *__restrict__ flags2)
+{
+    static_assert(CW * CH < 16, "active-tile count fits 4 bits");
+    const uint32_t t = *step_base + step_add;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (t % SKIP_PERIOD != 0 || c >= nchunks)
+        return;
+    const uint32_t q = t / SKIP_PERIOD;
+    const uint32_t tag = q % LIVE_TAGS + 1;
+    const uint32_t *set = stat + (size_t)(t & 1u) * (size_t)stat_tiles;  // input set of step t
+    const int4 d = chunks[c];
+    const StepArgs<R> &a = members[d.x];
+    uint8_t f = (uint8_t)(tag << 5 | (uint32_t)(CW * CH) << 1 | 1u);
+    if (a.stat_in != nullptr)
+        f = chunk_flag<CW, CH>(set + a.tile_base, a.tiles_x, a.tiles_y, a.nx, a.ny, a.boundary,
+                               d.y, d.z, tag);
+    flags2[(size_t)(q & 1u) * (size_t)nchunks + c] = f;
+}
+
+// Exclusive prefix sum of v over the 1024 threads of a block; *total gets
+// the sum.  Every thread must call it (two barriers).
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int &total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[w] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int s = wsum[i];
+        off += i < w ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + x - v;
+}
+
+// One block of 1024 threads: the listed table from the answer of request
+// q - 1, as the host builds it -- live chunks by active-tile count, most
+// first (a stable order within a count), the j-th chunk dealt to XCD j % 8
+// at position j / 8 of that XCD's sequence, cw * ch entries per chunk
+// (tiles outside the member and the last round's missing chunks are padding
+// entries, member -1).  *count = entries, or -1: no valid answer, or every
+// chunk live (the full table).  *periods counts the lists made.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_batch_compact(const StepArgs<R> *__restrict__ members,
+                                                        const int4 *__restrict__ chunks,
+                                                        int nchunks, int cw, int ch,
+                                                        const uint32_t *__restrict__ step_base,
+                                                        uint32_t step_add,
+                                                        const uint8_t *__restrict__ flags2,
+                                                        int4 *__restrict__ out,
+                                                        int *__restrict__ count,
+                                                        uint32_t *__restrict__ periods)
+{
+    constexpr int FLAGS_LDS = 49152;  // answers of up to this many chunks are read from LDS
+    __shared__ int wsum[16];
+    __shared__ uint8_t sflags[FLAGS_LDS];
+    const uint32_t t = *step_base + step_add;
+    if (t % SKIP_PERIOD != 0)
+        return;
+    const uint32_t q = t / SKIP_PERIOD;
+    const int tid = threadIdx.x;
+    if (q == 0) {
+        if (tid == 0)
+            *count = -1;
+        return;
+    }
+    const uint32_t want_tag = (q - 1) % LIVE_TAGS + 1;
+    const uint8_t *flags = flags2 + (size_t)((q - 1) & 1u) * (size_t)nchunks;
+    if (nchunks <= FLAGS_LDS) {  // one coalesced read; the passes below read LDS
+        for (int c = tid; c < nchunks; c += 1024)
+            sflags[c] = flags[c];
+        __syncthreads();
+        flags = sflags;
+    }
+    const int per = (nchunks + 1023) / 1024;
+    const int lo = min(tid * per, nchunks), hi = min(lo + per, nchunks);
+    int bad = 0;
+    for (int c = lo; c < hi; ++c)
+        bad |= (uint32_t)(flags[c] >> 5) != want_tag;
+    if (__syncthreads_or(bad)) {
+        if (tid == 0)
+            *count = -1;
+        return;
+    }
+    const int area = cw * ch;
+    int base = 0;  // chunks placed so far (the same in every thread)
+    for (int na = area; na >= 0; --na) {
+        const uint32_t want = (uint32_t)na << 1 | 1u;
+        int n = 0;
+        for (int c = lo; c < hi; ++c)
+            n += (flags[c] & 31u) == want ? 1 : 0;
+        int total = 0;
+        int j = base + block_excl_scan(n, wsum, total);
+        for (int c = lo; c < hi; ++c) {
+            if ((flags[c] & 31u) != want)
+                continue;
+            const int4 d = chunks[c];
+            const StepArgs<R> &a = members[d.x];
+            const int tx = (a.nx + TX - 1) / TX, ty = (a.ny + TY - 1) / TY;
+            int4 *o = out + (size_t)(j / 8) * area * 8 + j % 8;
+            for (int k = 0; k < area; ++k) {
+                const int x = d.y * cw + k % cw, y = d.z * ch + k / cw;
+                o[(size_t)k * 8] = x < tx && y < ty ? make_int4(d.x, x * TX, y * TY, 0)
+                                                    : make_int4(-1, 0, 0, 0);
+            }
+            ++j;
+        }
+        base += total;
+    }
+    // the last round's missing chunks: padding
+    const int rounded = (base + 7) / 8 * 8;
+    for (int i = tid; i < (rounded - base) * area; i += 1024) {
+        const int jj = base + i / area, k = i % area;
+        out[((size_t)(jj / 8) * area + k) * 8 + jj % 8] = make_int4(-1, 0, 0, 0);
+    }
+    if (tid == 0) {
+        const bool all = base == nchunks;
+        *count = all ? -1 : rounded * area;
+        if (!all)
+            *periods += 1u;
+    }
+}
+
 template <typename R, int BC>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a, LiveReq q)
 {
@@ -1201,17 +1352,33 @@ static_assert(2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_Y * TY &&
 // is *step_base + step_add (the noise counter); its parity is the ping-pong
 // direction (a batch always restarts at step 0 in buffer 0), so a captured
 // graph replays correctly after its k_advance node moves *step_base on.
-template <typename R>
+// DL: the table is chosen on the device -- the listed table `listed` of
+// *listed_n entries when *listed_n >= 0 (built by k_batch_compact), else the
+// full one; the grid covers the longer of the two, and both entries are
+// loaded together with the count so the choice adds no round trip.
+template <typename R, bool DL>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_batch(const StepArgs<R> *__restrict__ members,
                                                    const int4 *__restrict__ tiles, int ntiles,
                                                    const uint32_t *__restrict__ step_base,
-                                                   uint32_t step_add)
+                                                   uint32_t step_add,
+                                                   const int4 *__restrict__ listed,
+                                                   const int *__restrict__ listed_n,
+                                                   int listed_cap)
 {
     __shared__ __align__(16) TileSmem<R> sm;
     const int t = blockIdx.x;
-    if (t >= ntiles)
-        return;
-    const int4 d = tiles[t];
+    int4 d;
+    if (DL) {
+        const int4 none = make_int4(-1, 0, 0, 0);
+        const int4 dl = t < listed_cap ? listed[t] : none;
+        const int4 df = t < ntiles ? tiles[t] : none;
+        const int n = *listed_n;
+        d = n >= 0 ? (t < n ? dl : none) : df;
+    } else {
+        if (t >= ntiles)
+            return;
+        d = tiles[t];
+    }
     if (d.x < 0)
         return;  // padding of the table
     StepArgs<R> a = members[d.x];
@@ -1660,10 +1827,43 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s)
 
 template <typename R>
 hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
-                             const uint32_t *step_base, uint32_t step_add, hipStream_t s)
+                             const uint32_t *step_base, uint32_t step_add, hipStream_t s,
+                             const DeviceLists *dl)
 {
-    hipLaunchKernelGGL((k_step_batch<R>), dim3(ntiles), dim3(NT), 0, s, members, tiles, ntiles,
-                       step_base, step_add);
+    if (dl == nullptr) {
+        hipLaunchKernelGGL((k_step_batch<R, false>), dim3(ntiles), dim3(NT), 0, s, members,
+                           tiles, ntiles, step_base, step_add, nullptr, nullptr, 0);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_batch_compact<R>), dim3(1), dim3(1024), 0, s, members, dl->chunks,
+                       dl->nchunks, dl->cw, dl->ch, step_base, step_add, dl->flags2, dl->listed,
+                       dl->count, dl->periods);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    const dim3 lg((dl->nchunks + 255) / 256), lb(256);
+#define KG_BATCH_SHAPE(W, H)                                                                   \
+    else if (dl->cw == W && dl->ch == H) hipLaunchKernelGGL((k_batch_live_dev<R, W, H>), lg, lb, \
+                                                            0, s, members, dl->chunks,          \
+                                                            dl->nchunks, dl->stat,              \
+                                                            dl->stat_tiles, step_base,          \
+                                                            step_add, dl->flags2);
+    if (dl->nchunks <= 0)
+        return hipErrorInvalidValue;
+    KG_BATCH_SHAPE(2, 2)
+    KG_BATCH_SHAPE(2, 4)
+    KG_BATCH_SHAPE(4, 2)
+    KG_BATCH_SHAPE(4, 1)
+    KG_BATCH_SHAPE(8, 1)
+    else
+        return hipErrorInvalidValue;  // not a shape of BATCH_SHAPES
+#undef KG_BATCH_SHAPE
+    e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    const int grid = std::max(ntiles, dl->listed_cap);
+    hipLaunchKernelGGL((k_step_batch<R, true>), dim3(grid), dim3(NT), 0, s, members, tiles,
+                       ntiles, step_base, step_add, dl->listed, dl->count, dl->listed_cap);
     return hipGetLastError();
 }
 
@@ -1795,7 +1995,8 @@ hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double 
                                              const uint32_t *, uint32_t, uint8_t *, int, int,     \
                                              hipStream_t);                                        \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
-                                             const uint32_t *, uint32_t, hipStream_t);           \
+                                             const uint32_t *, uint32_t, hipStream_t,            \
+                                             const DeviceLists *);                               \
     template hipError_t launch_step_batch_direct<R>(const StepArgs<R> *, const int4 *, int,      \
                                                     uint32_t, hipStream_t);                      \
     template hipError_t launch_step_two_pass<R>(const StepArgs<R> &, const PassTemps<R> &,       \

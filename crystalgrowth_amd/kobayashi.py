@@ -14,8 +14,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (KG_BATCH_GRAPH, KG_BATCH_LAUNCH, KG_F32, KG_F64, KG_KERNEL_FUSED,
-                   KG_KERNEL_FUSED_DENSE, KG_KERNEL_TWO_PASS, Grads, Params, Stats, check, lib)
+from ._lib import (KG_BATCH_GRAPH, KG_BATCH_GRAPH_LISTS, KG_BATCH_LAUNCH, KG_F32, KG_F64,
+                   KG_KERNEL_FUSED, KG_KERNEL_FUSED_DENSE, KG_KERNEL_TWO_PASS, Grads, Params, Stats, check, lib)
 
 _PREC = {"f32": KG_F32, "float32": KG_F32, "f64": KG_F64, "float64": KG_F64}
 _KERNELS = {"fused": KG_KERNEL_FUSED, "two_pass": KG_KERNEL_TWO_PASS,
@@ -268,6 +268,10 @@ class Kobayashi(Solver):
         return self.colormap(order=1)
 
 
+_MODE_NAMES = {KG_BATCH_LAUNCH: "launch", KG_BATCH_GRAPH: "graph",
+               KG_BATCH_GRAPH_LISTS: "graph_lists"}
+
+
 class Batch:
     """An ensemble of independent grids of any sizes and parameters stepped
     together (kg_batch_* of include/kobayashi_gpu.h).  Member m evolves bit for
@@ -378,13 +382,14 @@ class Batch:
         return [g.as_dict() for g in out], dT
 
     def set_mode(self, mode: str = "launch", graph_steps: int = 0) -> None:
-        m = {"launch": KG_BATCH_LAUNCH, "graph": KG_BATCH_GRAPH}[mode]
+        m = {"launch": KG_BATCH_LAUNCH, "graph": KG_BATCH_GRAPH,
+             "graph_lists": KG_BATCH_GRAPH_LISTS}[mode]
         check(lib().kg_batch_set_mode(self._h, m, int(graph_steps)))
 
     def mode(self):
         m, g = ctypes.c_int(), ctypes.c_int()
         check(lib().kg_batch_get_mode(self._h, ctypes.byref(m), ctypes.byref(g)))
-        return ("launch" if m.value == KG_BATCH_LAUNCH else "graph"), g.value
+        return _MODE_NAMES[m.value], g.value
 
     CHUNK_SHAPES = ((2, 2), (2, 4), (4, 2), (4, 1), (8, 1))
 
@@ -405,7 +410,7 @@ class Batch:
         form (chunk_shape gives the shape)."""
         m, g = ctypes.c_int(), ctypes.c_int()
         check(lib().kg_batch_autotune(self._h, int(trial_steps), ctypes.byref(m), ctypes.byref(g)))
-        return ("launch" if m.value == KG_BATCH_LAUNCH else "graph"), g.value
+        return _MODE_NAMES[m.value], g.value
 
     def step(self, n: int = 1) -> None:
         check(lib().kg_batch_step(self._h, int(n)))

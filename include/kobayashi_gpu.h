@@ -223,6 +223,7 @@ int kg_adjoint(kg_handle h, int nsteps, int checkpoint_every, const void *target
  * per-member semantics are those of kg_step. */
 #define KG_BATCH_LAUNCH 0         /* one kernel launch per step */
 #define KG_BATCH_GRAPH 1          /* a captured hipGraph of graph_steps launches per replay */
+#define KG_BATCH_GRAPH_LISTS 2    /* the same, with the chunk lists kept on the device */
 
 typedef struct kg_batch_s *kg_batch;
 
@@ -245,7 +246,12 @@ int kg_batch_reduce(kg_batch b, int member, kg_stats *out);
 /* kg_front_profile of one member */
 int kg_batch_front_profile(kg_batch b, int member, double cx, double cy, int n, double level,
                            double *radii);
-/* mode KG_BATCH_LAUNCH or KG_BATCH_GRAPH (graph_steps even, >= 2) */
+/* mode KG_BATCH_LAUNCH, KG_BATCH_GRAPH or KG_BATCH_GRAPH_LISTS (graph_steps
+ * even, >= 2).  KG_BATCH_LAUNCH skips chunks by lists the host makes
+ * (waiting every 7 steps for the GPU's answer); KG_BATCH_GRAPH steps every
+ * tile and never waits; KG_BATCH_GRAPH_LISTS never waits and skips chunks by
+ * lists made on the device, at the price of a fixed grid whose unused
+ * blocks exit and two small kernels per step. */
 int kg_batch_set_mode(kg_batch b, int mode, int graph_steps);
 int kg_batch_get_mode(kg_batch b, int *mode, int *graph_steps);
 /* Chunk shape (in 64 x 30 tiles) into which every member is cut: the step
@@ -266,7 +272,8 @@ int kg_batch_step_timed(kg_batch b, int nsteps, double *ms);
 int kg_batch_synchronize(kg_batch b);
 int kg_batch_get_step_count(kg_batch b, uint64_t *steps);
 /* Diagnostic, no counterpart in the reference: as kg_get_skip_stats, for a
- * batch in KG_BATCH_LAUNCH mode with at least 256 chunks over its members. */
+ * batch with at least 256 chunks over its members.  A list made on the
+ * device (KG_BATCH_GRAPH_LISTS) counts as the 7 launches it serves. */
 int kg_batch_get_skip_stats(kg_batch b, uint64_t *listed_launches, uint64_t *stale_answers);
 /* kg_adjoint for every member at once: targets[m] and d_T0[m] (optional;
  * the array or an entry may be NULL) are host fields of member m; out[m]

@@ -114,7 +114,7 @@ def test_autotune_restores_state(kb):
         b.step(3)
         before = [b.fields(m) for m in range(len(sizes))]
         mode, g = b.autotune(8)
-        assert mode in ("launch", "graph") and (mode == "launch" or g >= 2)
+        assert mode in ("launch", "graph", "graph_lists") and (mode == "launch" or g >= 2)
         assert b.chunk_shape in kb.Batch.CHUNK_SHAPES
         assert b.step_count == 3
         for m in range(len(sizes)):
@@ -162,6 +162,32 @@ def test_member_with_wide_stored_angles(kb):
     ref = plain(kb, nx, ny, "f32", p, (phi, T, th), 20)
     for x, y in zip(got, ref):
         assert np.array_equal(x, y)
+
+
+def list_batch(kb):
+    sizes = [(1024, 960), (770, 610), (1030, 901), (640, 480), (900, 905)]
+    params = [
+        kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1, seed=1, instance=0),
+        kb.default_params(delta=0.02, aniso=4.0, noise_a=0.02, boundary=0, seed=2, instance=1),
+        kb.default_params(delta=0.05, noise_a=0.01, boundary=0, seed=3, instance=2),
+        kb.default_params(aniso=5.0, theta0=0.7, noise_a=0.01, boundary=1, seed=4, instance=3),
+        kb.default_params(delta=0.06, K=0.9, noise_a=0.01, boundary=0, seed=5, instance=4),
+    ]
+    spots = [(100, 120), (766, 3), (5, 895), (320, 240), (899, 450)]
+    return sizes, params, spots
+
+
+def check_against_dense(kb, prec, sizes, params, spots, got, marks):
+    for m, ((nx, ny), p, (cx, cy)) in enumerate(zip(sizes, params, spots)):
+        with kb.Solver(nx, ny, prec, p) as s:
+            s.set_kernel("fused_dense")
+            s.reset_disk(cx, cy, 6.0)
+            done = 0
+            for mk in marks:
+                s.step(mk - done)
+                done = mk
+                for name, x, y in zip(("phi", "T", "theta"), got[mk][m], s.fields()):
+                    assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), (m, mk, name)
 
 
 @pytest.mark.parametrize("prec,shape", [("f32", (2, 2)), ("f64", (2, 2)), ("f32", (2, 4)),
@@ -232,3 +258,55 @@ def test_chunk_shape_changes_mid_run(kb):
             ref = plain(kb, nx, ny, "f32", params[m], init[m], done)
             for x, y in zip(b.fields(m), ref):
                 assert np.array_equal(x, y), m
+
+
+@pytest.mark.parametrize("prec,shape,g", [("f32", (2, 2), 2), ("f32", (8, 1), 8),
+                                          ("f64", (2, 2), 8)])
+def test_graph_mode_device_lists_match_plain_solver_bitwise(kb, prec, shape, g):
+    """KG_BATCH_GRAPH_LISTS over a batch of 256+ chunks keeps its chunk
+    lists on the device (request and list kernels driven by the device step
+    counter, no host wait): replays of a captured graph, and the steps
+    between whole replays, leave every member bit for bit on a dense plain
+    handle's trajectory, and the lists are used."""
+    sizes, params, spots = list_batch(kb)
+    marks = [1, 9, 40, 151, 400]
+    with kb.Batch(sizes, params, prec) as b:
+        b.set_chunk_shape(*shape)
+        b.set_mode("graph_lists", g)
+        assert b.mode() == ("graph_lists", g)
+        for m, (cx, cy) in enumerate(spots):
+            b.reset_disk(m, cx, cy, 6.0)
+        got, done = {}, 0
+        for mk in marks:
+            b.step(mk - done)
+            done = mk
+            got[mk] = [b.fields(m) for m in range(len(sizes))]
+        stats = b.skip_stats()
+    check_against_dense(kb, prec, sizes, params, spots, got, marks)
+    # one list per 7 steps from step 7 on, each counted as 7 launches
+    assert stats["listed_launches"] >= 350 and stats["stale_answers"] == 0, stats
+
+
+def test_lists_across_mode_and_shape_changes(kb):
+    """Host lists (launch mode), device lists and no lists (the graph
+    modes) in turn, with chunk-shape changes, mid-period switches and
+    replays of different lengths: every member stays bit for bit on a dense
+    plain handle's trajectory."""
+    sizes, params, spots = list_batch(kb)
+    plan = [("launch", 0, (2, 2), 23), ("graph_lists", 2, (2, 2), 30),
+            ("graph_lists", 8, (4, 1), 41), ("graph", 4, (4, 1), 17), ("launch", 0, (4, 1), 12),
+            ("graph_lists", 32, (2, 4), 70), ("graph", 8, (2, 4), 16),
+            ("graph_lists", 8, (2, 4), 29), ("launch", 0, (2, 2), 9)]
+    marks, done = [], 0
+    with kb.Batch(sizes, params, "f32") as b:
+        for m, (cx, cy) in enumerate(spots):
+            b.reset_disk(m, cx, cy, 6.0)
+        got = {}
+        for mode, g, shape, n in plan:
+            b.set_chunk_shape(*shape)
+            b.set_mode(mode, g)
+            b.step(n)
+            done += n
+            marks.append(done)
+            got[done] = [b.fields(m) for m in range(len(sizes))]
+    check_against_dense(kb, "f32", sizes, params, spots, got, marks)
