@@ -1003,6 +1003,9 @@ template <> struct Occupancy<double> { static constexpr int waves = 1; };
 // Tile of tile block b when the launch covers only the chunks of `live`:
 // the j-th listed chunk goes to XCD j % 8 and its 8 tiles to consecutive
 // blocks of that XCD, as chunk_tile deals all chunks.
+// DEV: the list is in device memory (live.dev), else in the kernel
+// arguments (live.c); two kernels, so neither carries the other's select.
+template <bool DEV>
 __device__ __forceinline__ bool listed_tile(const LiveList &live, int b, int tiles_x, int tiles_y,
                                             int &tx, int &ty)
 {
@@ -1012,7 +1015,7 @@ __device__ __forceinline__ bool listed_tile(const LiveList &live, int b, int til
         return false;
     // the 32-bit word holding entry j (a scalar load; there is no 16-bit one)
     const uint32_t w =
-        live.dev ? live.dev[j >> 1] : reinterpret_cast<const uint32_t *>(live.c)[j >> 1];
+        DEV ? live.dev[j >> 1] : reinterpret_cast<const uint32_t *>(live.c)[j >> 1];
     const int c = (int)((j & 1) ? w >> 16 : w & 0xffffu);
     const int k = i % (CHUNK_W * CHUNK_H);
     const int ncx = (tiles_x + CHUNK_W - 1) / CHUNK_W;
@@ -1065,35 +1068,40 @@ __device__ __forceinline__ uint8_t chunk_flag(const uint32_t *__restrict__ sa, i
 {
     constexpr uint32_t Q = S_PHI0 | S_T0;
     constexpr int LIVE_W = CW + 2 * SKIP_MARGIN_X + 2, LIVE_H = CH + 2 * SKIP_MARGIN_Y + 2;
-    int col[LIVE_W], row[LIVE_H];
-#pragma unroll
-    for (int i = 0; i < LIVE_W; ++i)
-        col[i] = live_wrap(cx * CW - SKIP_MARGIN_X - 1 + i, tiles_x, boundary);
-#pragma unroll
-    for (int j = 0; j < LIVE_H; ++j)
-        row[j] = live_wrap(cy * CH - SKIP_MARGIN_Y - 1 + j, tiles_y, boundary);
+    // Window columns cx * CW - SKIP_MARGIN_X - 1 + i, i < LIVE_W, and rows
+    // likewise, mapped by the boundary; the outermost column (row) on each
+    // side reaches one tile further only where the tile next to it is a
+    // narrow periodic edge tile, and otherwise repeats its neighbour.
+    // Indices are computed as used (no index arrays: they cost the step
+    // kernels that carry this request register spills).
     const int wx = nx - (tiles_x - 1) * TX, wy = ny - (tiles_y - 1) * TY;
     const bool narrow_x = boundary == 0 && wx < LIVE_NEED;
     const bool narrow_y = boundary == 0 && wy < LIVE_NEED;
-    if (!(narrow_x && col[1] == tiles_x - 1))
-        col[0] = col[1];
-    if (!(narrow_x && col[LIVE_W - 2] == tiles_x - 1))
-        col[LIVE_W - 1] = col[LIVE_W - 2];
-    if (!(narrow_y && row[1] == tiles_y - 1))
-        row[0] = row[1];
-    if (!(narrow_y && row[LIVE_H - 2] == tiles_y - 1))
-        row[LIVE_H - 1] = row[LIVE_H - 2];
+    const int x_1 = cx * CW - SKIP_MARGIN_X - 1, y_1 = cy * CH - SKIP_MARGIN_Y - 1;
+    const int c1 = live_wrap(x_1 + 1, tiles_x, boundary);
+    const int cl = live_wrap(x_1 + LIVE_W - 2, tiles_x, boundary);
+    const int r1 = live_wrap(y_1 + 1, tiles_y, boundary);
+    const int rl = live_wrap(y_1 + LIVE_H - 2, tiles_y, boundary);
+    const int col0 = narrow_x && c1 == tiles_x - 1 ? live_wrap(x_1, tiles_x, boundary) : c1;
+    const int coll =
+        narrow_x && cl == tiles_x - 1 ? live_wrap(x_1 + LIVE_W - 1, tiles_x, boundary) : cl;
+    const int row0 = narrow_y && r1 == tiles_y - 1 ? live_wrap(y_1, tiles_y, boundary) : r1;
+    const int rowl =
+        narrow_y && rl == tiles_y - 1 ? live_wrap(y_1 + LIVE_H - 1, tiles_y, boundary) : rl;
     uint32_t any = 0, act = 0;  // act: tiles of the chunk that took the full path
-#pragma unroll
-    for (int j = 0; j < LIVE_H; ++j)
-#pragma unroll
+    for (int j = 0; j < LIVE_H; ++j) {
+        const int rr = j == 0 ? row0
+                              : (j == LIVE_H - 1 ? rowl : live_wrap(y_1 + j, tiles_y, boundary));
         for (int i = 0; i < LIVE_W; ++i) {
-            const uint32_t w = sa[row[j] * tiles_x + col[i]];
+            const int cc = i == 0 ? col0
+                                  : (i == LIVE_W - 1 ? coll : live_wrap(x_1 + i, tiles_x, boundary));
+            const uint32_t w = sa[rr * tiles_x + cc];
             any |= w ^ Q;
             if (j > SKIP_MARGIN_Y && j <= SKIP_MARGIN_Y + CH && i > SKIP_MARGIN_X &&
                 i <= SKIP_MARGIN_X + CW)
                 act += (w & S_ACTIVE) ? 1u : 0u;
         }
+    }
     return (uint8_t)(tag << 5 | act << 1 | (any ? 1u : 0u));
 }
 
@@ -1314,7 +1322,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 
 // The same step over the chunks of `live` only (a separate kernel, so that
 // launches of every chunk keep their small kernel arguments).
-template <typename R, int BC>
+template <typename R, int BC, bool DEV>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused_listed(StepArgs<R> a, LiveList live, LiveReq q)
 {
     __shared__ __align__(16) TileSmem<R> sm;
@@ -1324,7 +1332,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
         return;
     }
     int tx, ty;
-    if (!listed_tile(live, (int)blockIdx.x - q.blocks, tiles_x, tiles_y, tx, ty))
+    if (!listed_tile<DEV>(live, (int)blockIdx.x - q.blocks, tiles_x, tiles_y, tx, ty))
         return;
     step_tile<R, BC>(a, tx * TX, ty * TY, sm);
 }
@@ -1700,12 +1708,19 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList
         const int nblocks = (live->n + NXCD - 1) / NXCD * NXCD * CHUNK_W * CHUNK_H + q.blocks;
         if (nblocks == 0)
             return hipSuccess;  // no chunk to step, nothing asked
-        if (a.boundary == 1)
-            hipLaunchKernelGGL((k_step_fused_listed<R, 1>), dim3(nblocks), dim3(NT), 0, s, b, *live,
-                               q);
+        const bool dev = live->dev != nullptr;
+        if (a.boundary == 1 && dev)
+            hipLaunchKernelGGL((k_step_fused_listed<R, 1, true>), dim3(nblocks), dim3(NT), 0, s, b,
+                               *live, q);
+        else if (a.boundary == 1)
+            hipLaunchKernelGGL((k_step_fused_listed<R, 1, false>), dim3(nblocks), dim3(NT), 0, s, b,
+                               *live, q);
+        else if (dev)
+            hipLaunchKernelGGL((k_step_fused_listed<R, 0, true>), dim3(nblocks), dim3(NT), 0, s, b,
+                               *live, q);
         else
-            hipLaunchKernelGGL((k_step_fused_listed<R, 0>), dim3(nblocks), dim3(NT), 0, s, b, *live,
-                               q);
+            hipLaunchKernelGGL((k_step_fused_listed<R, 0, false>), dim3(nblocks), dim3(NT), 0, s, b,
+                               *live, q);
         return hipGetLastError();
     }
     if (!a.stat_in) {
