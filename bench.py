@@ -93,6 +93,10 @@ def parse():
                     help="Adam iterations of the batched fit (0 skips it)")
     ap.add_argument("--no-looped", action="store_true",
                     help="batched leg without the looped plain handles (for profiling)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched leg in launch mode with 2 x 2 chunks, the tuner's pick on this "
+                         "ensemble, without running the tuner (for profiling: the tuner's graph "
+                         "candidates add captured launches to the trace)")
     ap.add_argument("--calibrate-copy", action="store_true")
     return ap.parse_args()
 
@@ -326,7 +330,7 @@ def ceilings(kb, size: int, launches: int = 50):
 TUNE_AT = 1000
 
 
-def batched_vs_looped(kb, nsteps: int, looped: bool = True):
+def batched_vs_looped(kb, nsteps: int, looped: bool = True, autotune: bool = True):
     """The ensemble from its initial state (disk r = 5 at every member's
     centre) for nsteps steps: one batched launch per step vs each member's
     own plain handle stepped in turn -- all queued at once (every handle has
@@ -340,9 +344,14 @@ def batched_vs_looped(kb, nsteps: int, looped: bool = True):
         # tuned on the run's own state at step TUNE_AT, where quiescent, flat
         # and active tiles are all present (at step 0 nearly every tile is
         # quiescent, which favours a shape that the later steps do not)
-        b.reset_disk(-1, -1, -1, 5.0)
-        b.step(TUNE_AT)
-        mode, g = b.autotune(32)          # restores the state it ran from
+        if autotune:
+            b.reset_disk(-1, -1, -1, 5.0)
+            b.step(TUNE_AT)
+            mode, g = b.autotune(32)      # restores the state it ran from
+        else:
+            b.set_chunk_shape(2, 2)
+            b.set_mode("launch")
+            mode, g = b.mode()
         shape = b.chunk_shape
         b.reset_disk(-1, -1, -1, 5.0)
         b.synchronize()
@@ -354,7 +363,7 @@ def batched_vs_looped(kb, nsteps: int, looped: bool = True):
     if not looped:
         return {"members": len(sizes), "cells": cells, "steps": nsteps,
                 "batched_Gcell_updates_per_s": round(cells * nsteps / tb / 1e9, 3),
-                "launch_mode": [mode, g], "chunk_shape": list(shape), "tuned_at_step": TUNE_AT,
+                "launch_mode": [mode, g], "chunk_shape": list(shape), "tuned_at_step": TUNE_AT if autotune else None,
                 "batched_s": round(tb, 3)}
     solvers = [kb.Solver(nx, ny, "f32", p) for (nx, ny), p in zip(sizes, params)]
     for (nx, ny), s in zip(sizes, solvers):
@@ -386,7 +395,7 @@ def batched_vs_looped(kb, nsteps: int, looped: bool = True):
             "speedup_vs_looped": round(tl / tb, 3),
             "looped_sequential_Gcell_updates_per_s": round(cells * nsteps / ts / 1e9, 3),
             "speedup_vs_looped_sequential": round(ts / tb, 3),
-            "launch_mode": [mode, g], "chunk_shape": list(shape), "tuned_at_step": TUNE_AT,
+            "launch_mode": [mode, g], "chunk_shape": list(shape), "tuned_at_step": TUNE_AT if autotune else None,
             "batched_s": round(tb, 3), "looped_s": round(tl, 3), "looped_sequential_s": round(ts, 3),
             "same_state": sb == sl,
             "note": "looped: every member's own handle and stream, all members queued at once "
@@ -533,7 +542,8 @@ def main():
     if args.horizon > 0:
         rec["trajectory"] = trajectory(kb, n, args.horizon)
     if args.batch_steps > 0:
-        rec["batched"] = batched_vs_looped(kb, args.batch_steps, not args.no_looped)
+        rec["batched"] = batched_vs_looped(kb, args.batch_steps, not args.no_looped,
+                                           not args.no_autotune)
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_vs_forward(kb)
     if args.fit_iterations > 0:
