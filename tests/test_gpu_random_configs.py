@@ -1,6 +1,6 @@
 """Randomised configurations against the CPU oracle (needs a GPU).
 
-Twenty-four configurations drawn from a fixed seed: grid sizes from 3 x 3
+Sixty-four configurations drawn from a fixed seed: grid sizes from 3 x 3
 to 259 x 199 (widths that are and are not multiples of 4, ragged last tiles),
 every parameter of the model drawn over and beyond the reference GUI's
 slider ranges (negative K, non-integer anisotropy, theta0 outside
@@ -11,13 +11,17 @@ status, skips) and the dense one.  The same bars as tests/test_gpu_parity.py:
   FP64 kernel vs FP64 oracle: max |diff| <= 1e-10 on phi and T; theta equal
       to 1e-10 modulo 2 pi where the stencil differences are >= 1e-3.
   FP32 kernel vs FP32 oracle after one step: max |diff| <= 2e-6.
+
+Twenty-four more drawn configurations form one FP64 batch (members of
+every drawn size, parameter set and boundary in one launch per step), each
+member held to the same 1e-10 against its own oracle run.
 """
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-N_CONFIGS = 24
+N_CONFIGS = 64
 
 
 @pytest.fixture(scope="module")
@@ -102,3 +106,20 @@ def test_fp32_random_config_one_step_matches_oracle(kb, O, k):
     phi, T, _ = run_gpu(kb, nx, ny, p32, s32, 1, step0, "f32", "fused")
     assert np.max(np.abs(phi - ref.phi)) <= 2e-6
     assert np.max(np.abs(T - ref.T)) <= 2e-6
+
+
+def test_fp64_random_batch_members_match_oracle(kb, O):
+    members = [draw(kb, O, 500 + k) for k in range(24)]
+    nsteps = 17
+    with kb.Batch([(m[0], m[1]) for m in members], [kb.Params(**m[2].as_dict()) for m in members],
+                  "f64") as b:
+        for i, m in enumerate(members):
+            st = m[3]
+            b.set_fields(i, st.phi.astype(np.float64), st.T.astype(np.float64),
+                         st.theta.astype(np.float64))
+        b.step(nsteps)
+        got = [b.fields(i) for i in range(len(members))]
+    for i, m in enumerate(members):
+        ref = O.run(m[3].astype(np.float64), m[2], nsteps, 0)
+        assert np.max(np.abs(got[i][0] - ref.phi)) <= 1e-10, i
+        assert np.max(np.abs(got[i][1] - ref.T)) <= 1e-10, i
