@@ -251,7 +251,8 @@ int kg_batch_front_profile(kg_batch b, int member, double cx, double cy, int n, 
  * (waiting every 7 steps for the GPU's answer); KG_BATCH_GRAPH steps every
  * tile and never waits; KG_BATCH_GRAPH_LISTS never waits and skips chunks by
  * lists made on the device, at the price of a fixed grid whose unused
- * blocks exit and two small kernels per step. */
+ * blocks exit and two small kernels per step.  Chunk lists need at least
+ * 256 chunks over the members; smaller batches step every tile. */
 int kg_batch_set_mode(kg_batch b, int mode, int graph_steps);
 int kg_batch_get_mode(kg_batch b, int *mode, int *graph_steps);
 /* Chunk shape (in 64 x 30 tiles) into which every member is cut: the step
