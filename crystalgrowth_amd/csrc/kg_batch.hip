@@ -265,6 +265,17 @@ int make_tables(const kg_batch_s &b, int cw, int ch, Tables &t)
     return KG_OK;
 }
 
+// The boundary every member has (0 periodic, 1 zero-flux), or 2 when they
+// differ: a uniform batch launches the step kernel with one tile body.
+int boundary_mode(const kg_batch_s &b)
+{
+    const int first = b.params[0].boundary;
+    for (int m = 1; m < b.count; ++m)
+        if (b.params[m].boundary != first)
+            return 2;
+    return first == 1 ? 1 : 0;
+}
+
 // The device lists' arguments (graph mode), or null without them.
 bool dev_lists(const kg_batch_s &b, kg::DeviceLists &dl)
 {
@@ -426,10 +437,10 @@ int launch_steps(kg_batch_s &b, int n)
             e = b.precision == KG_F64
                     ? kg::launch_step_batch<double>(
                           static_cast<const kg::StepArgs<double> *>(b.members), tiles, nt,
-                          b.step_base, add, b.stream, dev ? &dl : nullptr)
+                          b.step_base, add, boundary_mode(b), b.stream, dev ? &dl : nullptr)
                     : kg::launch_step_batch<float>(
                           static_cast<const kg::StepArgs<float> *>(b.members), tiles, nt,
-                          b.step_base, add, b.stream, dev ? &dl : nullptr);
+                          b.step_base, add, boundary_mode(b), b.stream, dev ? &dl : nullptr);
         if (e != hipSuccess)
             return hip_fail(e, "batch step launch");
         if (listed)
@@ -451,10 +462,10 @@ int capture_graph(kg_batch_s &b)
         e = b.precision == KG_F64
                 ? kg::launch_step_batch<double>(static_cast<const kg::StepArgs<double> *>(b.members),
                                                 b.tiles, b.ntiles, b.step_base, (uint32_t)s,
-                                                b.stream, pdl)
+                                                boundary_mode(b), b.stream, pdl)
                 : kg::launch_step_batch<float>(static_cast<const kg::StepArgs<float> *>(b.members),
                                                b.tiles, b.ntiles, b.step_base, (uint32_t)s,
-                                               b.stream, pdl);
+                                               boundary_mode(b), b.stream, pdl);
     if (e == hipSuccess)
         e = kg::launch_advance(b.step_base, (uint32_t)b.graph_steps, b.stream);
     hipGraph_t g = nullptr;
@@ -664,7 +675,10 @@ int kg_batch_set_params(kg_batch b, int member, const kg_params *p)
         return rc;
     KG_HIP(hipSetDevice(b->device));
     KG_HIP(hipStreamSynchronize(b->stream));
+    const int bcm = boundary_mode(*b);
     b->params[member] = *p;
+    if (boundary_mode(*b) != bcm)
+        drop_graph(b);  // it launches the step kernel compiled for the old boundary mix
     invalidate_status(*b);
     hipError_t e;
     if (b->precision == KG_F64) {

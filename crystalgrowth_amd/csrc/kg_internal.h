@@ -195,13 +195,14 @@ struct DeviceLists {
     int stat_tiles;
 };
 // Batched step over a tile table (member, x0, y0); the step counter is
-// *step_base + step_add and its parity picks the ping-pong direction.  With
-// dl, the launch is preceded by the device lists' list and request kernels
+// *step_base + step_add and its parity picks the ping-pong direction.  bcm:
+// the boundary every member has (0, 1), or 2 when they differ.  With dl,
+// the launch is preceded by the device lists' list and request kernels
 // (both return at once except at multiples of SKIP_PERIOD) and uses the
 // listed table when there is one.
 template <typename R>
 hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
-                             const uint32_t *step_base, uint32_t step_add, hipStream_t s,
+                             const uint32_t *step_base, uint32_t step_add, int bcm, hipStream_t s,
                              const DeviceLists *dl = nullptr);
 // Batched step with the members' pointers as given and an absolute step.
 template <typename R>

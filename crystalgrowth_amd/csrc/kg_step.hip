@@ -1364,7 +1364,10 @@ static_assert(2 * (SKIP_AGE + 1) * SKIP_PERIOD + 2 <= SKIP_MARGIN_Y * TY &&
 // *listed_n entries when *listed_n >= 0 (built by k_batch_compact), else the
 // full one; the grid covers the longer of the two, and both entries are
 // loaded together with the count so the choice adds no round trip.
-template <typename R, bool DL>
+// BCM: the boundary of every member (0 periodic, 1 zero-flux), or 2 for a
+// batch that mixes them (both tile bodies compiled in, picked per member:
+// twice the code, and about 30 more scalar registers spilled to VGPR lanes).
+template <typename R, bool DL, int BCM>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_batch(const StepArgs<R> *__restrict__ members,
                                                    const int4 *__restrict__ tiles, int ntiles,
                                                    const uint32_t *__restrict__ step_base,
@@ -1401,7 +1404,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
         a.stat_in = a.stat_out;
         a.stat_out = const_cast<uint32_t *>(si);
     }
-    if (a.boundary == 1)
+    if (BCM == 1 || (BCM == 2 && a.boundary == 1))
         step_tile<R, 1>(a, d.y, d.z, sm);
     else
         step_tile<R, 0>(a, d.y, d.z, sm);
@@ -1840,16 +1843,32 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s)
     return hipGetLastError();
 }
 
+template <typename R, bool DL>
+hipError_t launch_step_batch_bc(int bcm, dim3 grid, hipStream_t s, const StepArgs<R> *members,
+                                const int4 *tiles, int ntiles, const uint32_t *step_base,
+                                uint32_t step_add, const int4 *listed, const int *listed_n,
+                                int listed_cap)
+{
+    if (bcm == 0)
+        hipLaunchKernelGGL((k_step_batch<R, DL, 0>), grid, dim3(NT), 0, s, members, tiles, ntiles,
+                           step_base, step_add, listed, listed_n, listed_cap);
+    else if (bcm == 1)
+        hipLaunchKernelGGL((k_step_batch<R, DL, 1>), grid, dim3(NT), 0, s, members, tiles, ntiles,
+                           step_base, step_add, listed, listed_n, listed_cap);
+    else
+        hipLaunchKernelGGL((k_step_batch<R, DL, 2>), grid, dim3(NT), 0, s, members, tiles, ntiles,
+                           step_base, step_add, listed, listed_n, listed_cap);
+    return hipGetLastError();
+}
+
 template <typename R>
 hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int ntiles,
-                             const uint32_t *step_base, uint32_t step_add, hipStream_t s,
+                             const uint32_t *step_base, uint32_t step_add, int bcm, hipStream_t s,
                              const DeviceLists *dl)
 {
-    if (dl == nullptr) {
-        hipLaunchKernelGGL((k_step_batch<R, false>), dim3(ntiles), dim3(NT), 0, s, members,
-                           tiles, ntiles, step_base, step_add, nullptr, nullptr, 0);
-        return hipGetLastError();
-    }
+    if (dl == nullptr)
+        return launch_step_batch_bc<R, false>(bcm, dim3(ntiles), s, members, tiles, ntiles,
+                                              step_base, step_add, nullptr, nullptr, 0);
     hipLaunchKernelGGL((k_batch_compact<R>), dim3(1), dim3(1024), 0, s, members, dl->chunks,
                        dl->nchunks, dl->cw, dl->ch, step_base, step_add, dl->flags2, dl->listed,
                        dl->count, dl->periods);
@@ -1877,9 +1896,8 @@ hipError_t launch_step_batch(const StepArgs<R> *members, const int4 *tiles, int 
     if (e != hipSuccess)
         return e;
     const int grid = std::max(ntiles, dl->listed_cap);
-    hipLaunchKernelGGL((k_step_batch<R, true>), dim3(grid), dim3(NT), 0, s, members, tiles,
-                       ntiles, step_base, step_add, dl->listed, dl->count, dl->listed_cap);
-    return hipGetLastError();
+    return launch_step_batch_bc<R, true>(bcm, dim3(grid), s, members, tiles, ntiles, step_base,
+                                         step_add, dl->listed, dl->count, dl->listed_cap);
 }
 
 template <typename R>
@@ -2010,7 +2028,7 @@ hipError_t launch_front_profile(const R *phi, int nx, int ny, double cx, double 
                                              const uint32_t *, uint32_t, uint8_t *, int, int,     \
                                              hipStream_t);                                        \
     template hipError_t launch_step_batch<R>(const StepArgs<R> *, const int4 *, int,             \
-                                             const uint32_t *, uint32_t, hipStream_t,            \
+                                             const uint32_t *, uint32_t, int, hipStream_t,       \
                                              const DeviceLists *);                               \
     template hipError_t launch_step_batch_direct<R>(const StepArgs<R> *, const int4 *, int,      \
                                                     uint32_t, hipStream_t);                      \

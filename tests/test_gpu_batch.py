@@ -310,3 +310,33 @@ def test_lists_across_mode_and_shape_changes(kb):
             marks.append(done)
             got[done] = [b.fields(m) for m in range(len(sizes))]
     check_against_dense(kb, "f32", sizes, params, spots, got, marks)
+
+
+@pytest.mark.parametrize("mode,g", [("launch", 0), ("graph", 4)])
+def test_uniform_boundary_batches_and_a_boundary_change(kb, mode, g):
+    """A batch whose members all share one boundary runs the step kernel
+    compiled for that boundary alone; a member switched to the other
+    boundary mid-run (a captured graph must then be rebuilt) keeps every
+    member bit for bit on its plain handle's trajectory."""
+    sizes = [(96, 64), (70, 90), (128, 40)]
+    params = [kb.default_params(delta=0.03 + 0.01 * k, aniso=4.0 + k, noise_a=0.01, boundary=0,
+                                seed=10 + k, instance=k) for k in range(3)]
+    init = initial(sizes, "f32")
+    with kb.Batch(sizes, params, "f32") as b:
+        b.set_mode(mode, g)
+        for m, st in enumerate(init):
+            b.set_fields(m, *st)
+        b.step(8)
+        changed = params[1].replace(boundary=1)
+        b.set_params(1, changed)
+        b.step(8)
+        got = [b.fields(m) for m in range(len(sizes))]
+    for m, (nx, ny) in enumerate(sizes):
+        with kb.Solver(nx, ny, "f32", params[m]) as s:
+            s.set_fields(*init[m])
+            s.step(8)
+            if m == 1:
+                s.params = changed
+            s.step(8)
+            for x, y in zip(got[m], s.fields()):
+                assert np.array_equal(x, y), m
