@@ -1322,19 +1322,34 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>
 
 // The same step over the chunks of `live` only (a separate kernel, so that
 // launches of every chunk keep their small kernel arguments).
-template <typename R, int BC, bool DEV>
+// REQ: the launch carries a liveness request in its first q.blocks blocks
+// (one launch in SKIP_PERIOD); the others use the kernel without that path,
+// which spills fewer scalar registers.
+template <typename R, int BC, bool DEV, bool REQ>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused_listed(StepArgs<R> a, LiveList live, LiveReq q)
 {
     __shared__ __align__(16) TileSmem<R> sm;
     const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
-    if ((int)blockIdx.x < q.blocks) {
+    if (REQ && (int)blockIdx.x < q.blocks) {
         live_request(a, q, tiles_x, tiles_y);
         return;
     }
     int tx, ty;
-    if (!listed_tile<DEV>(live, (int)blockIdx.x - q.blocks, tiles_x, tiles_y, tx, ty))
+    if (!listed_tile<DEV>(live, (int)blockIdx.x - (REQ ? q.blocks : 0), tiles_x, tiles_y, tx, ty))
         return;
     step_tile<R, BC>(a, tx * TX, ty * TY, sm);
+}
+
+template <typename R, int BC, bool DEV>
+void launch_listed(dim3 grid, hipStream_t s, const StepArgs<R> &b, const LiveList &live,
+                   const LiveReq &q)
+{
+    if (q.blocks > 0)
+        hipLaunchKernelGGL((k_step_fused_listed<R, BC, DEV, true>), grid, dim3(NT), 0, s, b, live,
+                           q);
+    else
+        hipLaunchKernelGGL((k_step_fused_listed<R, BC, DEV, false>), grid, dim3(NT), 0, s, b, live,
+                           q);
 }
 
 // A chunk skipped by a launch must stay all +0 for the launch.  The host uses
@@ -1713,17 +1728,13 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList
             return hipSuccess;  // no chunk to step, nothing asked
         const bool dev = live->dev != nullptr;
         if (a.boundary == 1 && dev)
-            hipLaunchKernelGGL((k_step_fused_listed<R, 1, true>), dim3(nblocks), dim3(NT), 0, s, b,
-                               *live, q);
+            launch_listed<R, 1, true>(dim3(nblocks), s, b, *live, q);
         else if (a.boundary == 1)
-            hipLaunchKernelGGL((k_step_fused_listed<R, 1, false>), dim3(nblocks), dim3(NT), 0, s, b,
-                               *live, q);
+            launch_listed<R, 1, false>(dim3(nblocks), s, b, *live, q);
         else if (dev)
-            hipLaunchKernelGGL((k_step_fused_listed<R, 0, true>), dim3(nblocks), dim3(NT), 0, s, b,
-                               *live, q);
+            launch_listed<R, 0, true>(dim3(nblocks), s, b, *live, q);
         else
-            hipLaunchKernelGGL((k_step_fused_listed<R, 0, false>), dim3(nblocks), dim3(NT), 0, s, b,
-                               *live, q);
+            launch_listed<R, 0, false>(dim3(nblocks), s, b, *live, q);
         return hipGetLastError();
     }
     if (!a.stat_in) {
