@@ -1291,17 +1291,18 @@ __global__ __launch_bounds__(1024) void k_batch_compact(const StepArgs<R> *__res
     }
 }
 
-template <typename R, int BC>
+// REQ as in k_step_fused_listed
+template <typename R, int BC, bool REQ>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(Occupancy<R>::waves))) void k_step_fused(StepArgs<R> a, LiveReq q)
 {
     __shared__ __align__(16) TileSmem<R> sm;
     const int tiles_x = (a.nx + TX - 1) / TX, tiles_y = (a.ny + TY - 1) / TY;
-    if ((int)blockIdx.x < q.blocks) {
+    if (REQ && (int)blockIdx.x < q.blocks) {
         live_request(a, q, tiles_x, tiles_y);
         return;
     }
     int tx, ty, c;
-    if (!chunk_tile((int)blockIdx.x - q.blocks, tiles_x, tiles_y, tx, ty, c))
+    if (!chunk_tile((int)blockIdx.x - (REQ ? q.blocks : 0), tiles_x, tiles_y, tx, ty, c))
         return;
     step_tile<R, BC>(a, tx * TX, ty * TY, sm);
 }
@@ -1746,10 +1747,14 @@ hipError_t launch_step_fused(const StepArgs<R> &a, hipStream_t s, const LiveList
         return hipGetLastError();
     }
     const int nblocks = chunk_blocks(tiles_x, tiles_y) + q.blocks;
-    if (a.boundary == 1)
-        hipLaunchKernelGGL((k_step_fused<R, 1>), dim3(nblocks), dim3(NT), 0, s, b, q);
+    if (a.boundary == 1 && q.blocks > 0)
+        hipLaunchKernelGGL((k_step_fused<R, 1, true>), dim3(nblocks), dim3(NT), 0, s, b, q);
+    else if (a.boundary == 1)
+        hipLaunchKernelGGL((k_step_fused<R, 1, false>), dim3(nblocks), dim3(NT), 0, s, b, q);
+    else if (q.blocks > 0)
+        hipLaunchKernelGGL((k_step_fused<R, 0, true>), dim3(nblocks), dim3(NT), 0, s, b, q);
     else
-        hipLaunchKernelGGL((k_step_fused<R, 0>), dim3(nblocks), dim3(NT), 0, s, b, q);
+        hipLaunchKernelGGL((k_step_fused<R, 0, false>), dim3(nblocks), dim3(NT), 0, s, b, q);
     return hipGetLastError();
 }
 
