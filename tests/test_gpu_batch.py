@@ -340,3 +340,25 @@ def test_uniform_boundary_batches_and_a_boundary_change(kb, mode, g):
             s.step(8)
             for x, y in zip(got[m], s.fields()):
                 assert np.array_equal(x, y), m
+
+
+@pytest.mark.parametrize("mode,g", [("graph", 8), ("graph_lists", 2)])
+def test_one_member_batch_is_a_replayable_single_grid(kb, mode, g):
+    """A batch of one member is the graph-replayable form of a single grid
+    (kg_step refuses stream capture): replays of its captured steps match a
+    plain handle bit for bit, including a step count that is not a whole
+    number of replays."""
+    nx, ny = 640, 480
+    p = kb.default_params(delta=0.04, aniso=6.0, K=2.0, theta0=0.2, noise_a=0.01, boundary=1,
+                          seed=3)
+    with kb.Batch([(nx, ny)], [p], "f32") as b:
+        b.set_mode(mode, g)
+        b.reset_seeds(4, 5.0, 8, 11, member=0)
+        b.step(45)
+        got = b.fields(0)
+    with kb.Solver(nx, ny, "f32", p) as s:
+        s.reset_seeds(4, 5.0, 8, 11)
+        s.step(45)
+        ref = s.fields()
+    for x, y in zip(got, ref):
+        assert np.array_equal(x, y)
