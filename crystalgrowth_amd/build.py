@@ -28,6 +28,10 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 (v_pk_*_f32) stays off: hand-written HIP must not rely on it
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the scheduler's AMDGPU register-pressure trackers: fewer scalar
+    # registers spilled in the step kernels, and the dense step 0.3-1.2 %
+    # faster (four A/B pairs on MI355X); the arithmetic is unchanged
+    "-mllvm", "-amdgpu-use-amdgpu-trackers=1",
     "-Wall", "-Wno-unused-function",
 ]
 
